@@ -1,0 +1,136 @@
+// Device-side building blocks shared by the MC and SMD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hsmd_common.h"
+#include "hsmd_internal.h"
+
+namespace hsmd {
+
+constexpr int kZMAX = 32;
+constexpr int kCMAX = 64;
+constexpr int kBCMAX = 16;
+
+#define HIP_CHECK(x)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess)                                                     \
+      throw ::hsmd::Error(HSMD_ERR_DEVICE, std::string(#x) + ": " +          \
+                                               hipGetErrorString(e_));        \
+  } while (0)
+
+// Owning device buffer (movable, not copyable).
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const T *h, size_t count) {
+    alloc(count);
+    if (count) HIP_CHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void zero() { if (n) HIP_CHECK(hipMemset(p, 0, n * sizeof(T))); }
+};
+
+// Couplings of the active neighbour slots of a lattice (new labels).
+// Slot s of site p reads neighbour nb[s*N + p]; the sentinel N holds a zero
+// spin, so missing neighbours add nothing.
+struct Coup {
+  int N, Npad, Z;
+  int general;               // 1: per-slot J table (disorder / Lieb)
+  int h_on;
+  double h;                  // external field (z for Heisenberg, Ising axis for Ising)
+  double jxy[kZMAX];         // uniform couplings per slot
+  double jz[kZMAX];          // uniform z couplings per slot (J * lambda)
+  double lam[kZMAX];         // lambda per slot (general mode)
+};
+
+struct Bonds {
+  int Nb;
+  int n_class;
+  double jxy[kBCMAX];
+  double jz[kBCMAX];
+};
+
+// h_i of the reference's get_localField (CF/subrouts.cpp:4465-4575):
+// start from -h on z, then add shells in slot order.
+template <typename T>
+__device__ __forceinline__ void local_field(const Coup &cp, const int32_t *__restrict__ nb,
+                                            const T *__restrict__ Jt, int p,
+                                            const T *sx, const T *sy, const T *sz,
+                                            T &hx, T &hy, T &hz) {
+  hx = T(0);
+  hy = T(0);
+  hz = cp.h_on ? T(-cp.h) : T(0);
+  const int N = cp.N;
+  if (!cp.general) {
+    for (int s = 0; s < cp.Z; s++) {
+      const int q = nb[(size_t)s * N + p];
+      const T J = T(cp.jxy[s]), Jz = T(cp.jz[s]);
+      hx += J * sx[q];
+      hy += J * sy[q];
+      hz += Jz * sz[q];
+    }
+  } else {
+    for (int s = 0; s < cp.Z; s++) {
+      const int q = nb[(size_t)s * N + p];
+      const T J = Jt[(size_t)s * N + p];
+      const T Jz = J * T(cp.lam[s]);
+      hx += J * sx[q];
+      hy += J * sy[q];
+      hz += Jz * sz[q];
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide FP64 sums of NV values; result valid in every thread.
+// scratch: at least NV * 16 doubles of shared memory.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) scratch[k * 16 + w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += scratch[k * 16 + i];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+}  // namespace hsmd

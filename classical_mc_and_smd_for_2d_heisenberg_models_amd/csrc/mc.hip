@@ -1,0 +1,960 @@
+// Monte Carlo sweep engine: heat-bath + overrelaxation (Heisenberg),
+// single-spin-flip Metropolis (Ising), on-device replica exchange.
+//
+// Reference hot path (one MPI rank per temperature):
+//   sweep loops            CF/mc_hm_code.cpp:2778-2962, 4836-4945
+//   overrelaxation         reflect_aboutVec  CF/subrouts.cpp:4956-5004
+//   heat bath              get_rotZ2VMat / get_hBathSample CF/subrouts.cpp:5095-5179
+//   Metropolis             CF/mc_hm_code.cpp:2872-2898
+//   energy                 get_energyValue   CF/subrouts.cpp:5181-5312
+//   replica exchange       CF/mc_hm_code.cpp:3015-3130, 5253-5365
+//   observables            CF/mc_hm_code.cpp:4951-5160, CF/subrouts.cpp:5362-5800
+//
+// Design: one workgroup per replica.  Sites are relabelled by colour class
+// (no two sites of a class share an active coupling), so a class is updated
+// in parallel and the classes are separated by workgroup barriers.  The
+// replica's spins live in LDS as SoA x/y/z arrays plus a zero sentinel site
+// that absorbs missing neighbours; the neighbour table (slot-major, shared by
+// all replicas) streams from L2.  The sweep epilogue reduces the FP64 energy
+// over the bond list and the order-parameter sums; a one-block kernel then
+// accumulates the observables per temperature slot and performs the
+// exchange by permuting slot labels (configurations never move).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "device_common.h"
+#include "hsmd_internal.h"
+
+namespace hsmd {
+
+constexpr int NOBS = 32;   // raw order-parameter sums per configuration
+constexpr int NMOM = HSMD_N_MOMENTS;
+constexpr int kThreads = 1024;
+
+struct SweepArgs {
+  Coup cp;
+  Bonds bd;
+  int n_colours;
+  int col_off[kCMAX + 1];
+  int n_or, n_hb, n_or_b, n_hb_b;
+  uint64_t seed;
+  uint32_t sweep;
+  int config_base;
+  int measure;
+  int n_stag, has_labels, z2, c3;
+  const int32_t *nb;
+  const void *Jt;
+  const int32_t *perm;
+  const int32_t *bi, *bj;
+  const uint8_t *bc;
+  const int8_t *lab;       // 4 x N: stag, hs, vs, ds (new labels)
+  const int32_t *z2nb;     // 3 x N new labels
+  void *spins;
+  const double *beta_slot;
+  const int32_t *slot_of_config;
+  const int32_t *boost_slot;
+  double *energy;          // per configuration
+  double *obs;             // per configuration, NOBS
+};
+
+// ------------------------------------------------------------ site updates
+
+// Overrelaxation: S' = 2 (S.h)/|h|^2 h - S; identity when |h|^2 <= 1e-12.
+template <typename T>
+__device__ __forceinline__ void overrelax(T hx, T hy, T hz, T &x, T &y, T &z) {
+  const T sq = hx * hx + hy * hy + hz * hz;
+  if (sq > T(1.0e-12)) {
+    const T fac = (T(2) / sq) * (x * hx + y * hy + z * hz);
+    x = fac * hx - x;
+    y = fac * hy - y;
+    z = fac * hz - z;
+  }
+}
+
+// FP64 heat bath, the reference's formulas: inverse-CDF cos(theta) about h,
+// rotation z -> h of get_rotZ2VMat (identity when h is along +-z).
+__device__ __forceinline__ void heat_bath_f64(double beta, double hx, double hy, double hz,
+                                              u32x4 r, double &x, double &y, double &z) {
+  const double u1 = u01_f64(r.x, r.y), u2 = u01_f64(r.z, r.w);
+  const double tht = 2.0 * M_PI * u1;
+  const double H = sqrt(hx * hx + hy * hy + hz * hz);
+  double chi;
+  if (fabs(H) > 1.0e-12) {
+    const double x0 = 1.0 + u2 * (exp(-2.0 * beta * H) - 1.0);
+    const double fc = -1.0 / (beta * H);
+    chi = fc * log(x0) - 1.0;
+  } else {
+    chi = 2.0 * u2 - 1.0;
+  }
+  const double w = sqrt(fmax(0.0, 1.0 - chi * chi));
+  double st, ct;
+  sincos(tht, &st, &ct);
+  const double s0 = w * ct, s1 = w * st, s2 = chi;
+  const double V0 = hx / H, V1 = hy / H, V2 = hz / H;
+  const double perp = V0 * V0 + V1 * V1;
+  if (perp > 0.0) {
+    const double sfac = (1.0 - V2) / perp;
+    const double x11 = 1.0 - sfac * V0 * V0;
+    const double x22 = 1.0 - sfac * V1 * V1;
+    const double x33 = x11 + x22 - 1.0;
+    const double x12 = V0 * V1 * (-sfac);
+    x = x11 * s0 + x12 * s1 + V0 * s2;
+    y = x12 * s0 + x22 * s1 + V1 * s2;
+    z = -V0 * s0 - V1 * s1 + x33 * s2;
+  } else {
+    x = s0;
+    y = s1;
+    z = s2;
+  }
+}
+
+// FP32 heat bath: same distribution, written for single precision.
+// t = 1 + cos(theta) = -log1p(u (e^{-2bH} - 1)) / (bH) keeps the pole region
+// accurate, and the frame about h is the branch-free orthonormal basis of
+// Duff et al. (JCGT 2017), equal to the reference rotation for h_z >= 0.
+__device__ __forceinline__ void heat_bath_f32(float beta, float hx, float hy, float hz,
+                                              u32x4 r, float &x, float &y, float &z) {
+  const float u1 = u01_f32(r.x), u2 = u01_f32(r.y);
+  const float H2 = hx * hx + hy * hy + hz * hz;
+  float t;
+  float ux = 0.f, uy = 0.f, uz = 1.f;
+  bool frame = false;
+  if (H2 > 1.0e-24f) {
+    const float H = sqrtf(H2);
+    const float bh = beta * H;
+    t = -log1pf(u2 * expm1f(-2.0f * bh)) / bh;
+    const float iH = 1.0f / H;
+    ux = hx * iH; uy = hy * iH; uz = hz * iH;
+    frame = true;
+  } else {
+    t = 2.0f * u2;
+  }
+  t = fminf(fmaxf(t, 0.0f), 2.0f);
+  const float chi = t - 1.0f;
+  const float w = sqrtf(t * (2.0f - t));
+  float st, ct;
+  __sincosf(6.2831853071795865f * u1, &st, &ct);
+  const float s0 = w * ct, s1 = w * st;
+  if (frame) {
+    const float sg = copysignf(1.0f, uz);
+    const float a = -1.0f / (sg + uz);
+    const float b = ux * uy * a;
+    // e1 = (1 + sg ux^2 a, sg b, -sg ux), e2 = (b, sg + uy^2 a, -uy)
+    const float e1x = 1.0f + sg * ux * ux * a, e1y = sg * b, e1z = -sg * ux;
+    const float e2x = b, e2y = sg + uy * uy * a, e2z = -uy;
+    x = s0 * e1x + s1 * e2x + chi * ux;
+    y = s0 * e1y + s1 * e2y + chi * uy;
+    z = s0 * e1z + s1 * e2z + chi * uz;
+  } else {
+    x = s0; y = s1; z = chi;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void heat_bath(double beta, T hx, T hy, T hz, u32x4 r, T &x, T &y, T &z);
+template <>
+__device__ __forceinline__ void heat_bath<double>(double beta, double hx, double hy, double hz,
+                                                  u32x4 r, double &x, double &y, double &z) {
+  heat_bath_f64(beta, hx, hy, hz, r, x, y, z);
+}
+template <>
+__device__ __forceinline__ void heat_bath<float>(double beta, float hx, float hy, float hz,
+                                                 u32x4 r, float &x, float &y, float &z) {
+  heat_bath_f32((float)beta, hx, hy, hz, r, x, y, z);
+}
+
+// ------------------------------------------------------- sweep epilogues
+
+// FP64 bond-list energy (get_energyValue) and order-parameter sums.
+template <typename T>
+__device__ void heis_epilogue(const SweepArgs &a, int c, const T *sx, const T *sy, const T *sz,
+                              double *scratch) {
+  const int N = a.cp.N;
+  double e = 0.0;
+  for (int b = threadIdx.x; b < a.bd.Nb; b += blockDim.x) {
+    const int i = a.bi[b], j = a.bj[b], k = a.bc[b];
+    e += a.bd.jxy[k] * ((double)sx[i] * (double)sx[j] + (double)sy[i] * (double)sy[j]) +
+         a.bd.jz[k] * ((double)sz[i] * (double)sz[j]);
+  }
+  double v[NOBS];
+#pragma unroll
+  for (int k = 0; k < NOBS; k++) v[k] = 0.0;
+  double mz = 0.0;
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
+    const double x = sx[p], y = sy[p], z = sz[p];
+    mz += z;
+    if (a.measure) {
+      v[0] += x; v[1] += y; v[2] += z;
+      if (a.has_labels) {
+        const int ls = a.lab[p], lh = a.lab[N + p], lv = a.lab[2 * N + p], ld = a.lab[3 * N + p];
+        if (ls >= 0) { v[3 + 3 * ls] += x; v[4 + 3 * ls] += y; v[5 + 3 * ls] += z; }
+        if (lh >= 0) { v[12 + 3 * lh] += x; v[13 + 3 * lh] += y; v[14 + 3 * lh] += z; }
+        if (lv >= 0) { v[18 + 3 * lv] += x; v[19 + 3 * lv] += y; v[20 + 3 * lv] += z; }
+        if (ld >= 0) { v[24 + 3 * ld] += x; v[25 + 3 * ld] += y; v[26 + 3 * ld] += z; }
+        if (a.z2) {
+          const int qx = a.z2nb[p], qy = a.z2nb[N + p], qxy = a.z2nb[2 * N + p];
+          const double d = (x - (double)sx[qxy]) * ((double)sx[qx] - (double)sx[qy]) +
+                           (y - (double)sy[qxy]) * ((double)sy[qx] - (double)sy[qy]) +
+                           (z - (double)sz[qxy]) * ((double)sz[qx] - (double)sz[qy]);
+          const double fc = (fabs(d) > 1.0e-12) ? 1.0 / fabs(d) : 0.25;
+          v[30] += fc * d;
+        }
+      }
+    }
+  }
+  v[31] = e - (a.cp.h_on ? a.cp.h * mz : 0.0);
+  block_sum<NOBS>(v, scratch);
+  if (threadIdx.x == 0) {
+    a.energy[c] = v[31];
+    if (a.measure)
+      for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
+  }
+}
+
+// ----------------------------------------------------------- Heisenberg
+
+template <typename T, bool LDS>
+__global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[NOBS * 16];
+  const int c = blockIdx.x + a.config_base;
+  const int N = a.cp.N, Npad = a.cp.Npad;
+  T *g = (T *)a.spins + (size_t)c * 3 * Npad;
+  T *s;
+  if (LDS) {
+    s = (T *)smem;
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) s[i] = g[i];
+    __syncthreads();
+  } else {
+    s = g;
+  }
+  T *sx = s, *sy = s + Npad, *sz = s + 2 * Npad;
+  const int slot = a.slot_of_config[c];
+  const double beta = a.beta_slot[slot];
+  const bool boost = a.boost_slot[slot] != 0;
+  const int nor = boost ? a.n_or_b : a.n_or;
+  const int nhb = boost ? a.n_hb_b : a.n_hb;
+  const T *Jt = (const T *)a.Jt;
+  for (int pass = 0; pass < nor + nhb; pass++) {
+    const bool hb = pass >= nor;
+    const uint32_t step = a.sweep * 16u + (uint32_t)pass;
+    for (int col = 0; col < a.n_colours; col++) {
+      for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
+        T hx, hy, hz;
+        local_field<T>(a.cp, a.nb, Jt, p, sx, sy, sz, hx, hy, hz);
+        T x = sx[p], y = sy[p], z = sz[p];
+        if (!hb) {
+          overrelax<T>(hx, hy, hz, x, y, z);
+        } else {
+          const u32x4 r = draw4(a.seed, (uint32_t)a.perm[p], step, (uint32_t)c, STREAM_MC_UPDATE);
+          heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
+        }
+        sx[p] = x; sy[p] = y; sz[p] = z;
+      }
+      __syncthreads();
+    }
+  }
+  heis_epilogue<T>(a, c, sx, sy, sz, scratch);
+  if (LDS) {
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) g[i] = s[i];
+  }
+}
+
+// --------------------------------------------------------------- Ising
+
+// Metropolis with FP64 fields and the deterministic exponential; spins are
+// int8 +-1 with a zero sentinel.  Energies use exact integer bond counts per
+// coupling class, E = sum_c J_c n_c - h M, evaluated in class order.
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[NOBS * 16];
+  __shared__ long long iscr[(kBCMAX + 1) * 16];
+  const int c = blockIdx.x + a.config_base;
+  const int N = a.cp.N, Npad = a.cp.Npad;
+  int8_t *g = (int8_t *)a.spins + (size_t)c * Npad;
+  int8_t *s;
+  if (LDS) {
+    s = (int8_t *)smem;
+    for (int i = threadIdx.x; i < Npad; i += blockDim.x) s[i] = g[i];
+    __syncthreads();
+  } else {
+    s = g;
+  }
+  const int slot = a.slot_of_config[c];
+  const double beta = a.beta_slot[slot];
+  const double *Jt = (const double *)a.Jt;
+  const uint32_t step = a.sweep * 16u;
+  for (int col = 0; col < a.n_colours; col++) {
+    for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
+      double h = a.cp.h_on ? -a.cp.h : 0.0;
+      for (int k = 0; k < a.cp.Z; k++) {
+        const int q = a.nb[(size_t)k * N + p];
+        const double J = a.cp.general ? Jt[(size_t)k * N + p] : a.cp.jxy[k];
+        h += J * (double)s[q];
+      }
+      const double spin = (double)s[p];
+      const double dE = 2.0 * spin * h;
+      const u32x4 r = draw4(a.seed, (uint32_t)a.perm[p], step, (uint32_t)c, STREAM_MC_UPDATE);
+      const double u = u01_f64(r.x, r.y);
+      if (accept_exp(u, beta * dE)) s[p] = (int8_t)(-s[p]);
+    }
+    __syncthreads();
+  }
+  // exact energy: integer bond counts per class
+  long long cnt[kBCMAX + 1];
+#pragma unroll
+  for (int k = 0; k <= kBCMAX; k++) cnt[k] = 0;
+  for (int b = threadIdx.x; b < a.bd.Nb; b += blockDim.x) {
+    const int k = a.bc[b];
+    const long long pr = (long long)(s[a.bi[b]] * s[a.bj[b]]);
+#pragma unroll
+    for (int kk = 0; kk < kBCMAX; kk++)
+      if (kk == k) cnt[kk] += pr;
+  }
+  for (int p = threadIdx.x; p < N; p += blockDim.x) cnt[kBCMAX] += s[p];
+  {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k <= kBCMAX; k++) cnt[k] = wave_sum_ll(cnt[k]);
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k <= kBCMAX; k++) iscr[k * 16 + w] = cnt[k];
+    __syncthreads();
+    for (int k = 0; k <= kBCMAX; k++) {
+      long long t = 0;
+      for (int i = 0; i < nw; i++) t += iscr[k * 16 + i];
+      cnt[k] = t;
+    }
+  }
+  // order parameters on spins (s, 0, 0)
+  double v[NOBS];
+#pragma unroll
+  for (int k = 0; k < NOBS; k++) v[k] = 0.0;
+  if (a.measure) {
+    for (int p = threadIdx.x; p < N; p += blockDim.x) {
+      const double x = s[p];
+      v[0] += x;
+      if (a.has_labels) {
+        const int ls = a.lab[p], lh = a.lab[N + p], lv = a.lab[2 * N + p], ld = a.lab[3 * N + p];
+        if (ls >= 0) v[3 + 3 * ls] += x;
+        if (lh >= 0) v[12 + 3 * lh] += x;
+        if (lv >= 0) v[18 + 3 * lv] += x;
+        if (ld >= 0) v[24 + 3 * ld] += x;
+        if (a.z2) {
+          const int qx = a.z2nb[p], qy = a.z2nb[N + p], qxy = a.z2nb[2 * N + p];
+          const double d = (x - (double)s[qxy]) * ((double)s[qx] - (double)s[qy]);
+          const double fc = (fabs(d) > 1.0e-12) ? 1.0 / fabs(d) : 0.25;
+          v[30] += fc * d;
+        }
+      }
+    }
+  }
+  block_sum<NOBS>(v, scratch);
+  if (threadIdx.x == 0) {
+    double E = 0.0;
+    for (int k = 0; k < a.bd.n_class; k++) E = fma(a.bd.jxy[k], (double)cnt[k], E);
+    if (a.cp.h_on) E = fma(-a.cp.h, (double)cnt[kBCMAX], E);
+    a.energy[c] = E;
+    if (a.measure)
+      for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
+  }
+  if (LDS) {
+    for (int i = threadIdx.x; i < Npad; i += blockDim.x) g[i] = s[i];
+  }
+}
+
+// ------------------------------------------- measurement + replica exchange
+
+struct PTArgs {
+  int R, N;
+  int parity;          // -1 none, 0/1 pair parity
+  int cap50;
+  int measure;
+  int n_stag, has_labels, z2, c3, h_on, ising;
+  uint64_t seed;
+  uint32_t sweep;
+  const double *energy;    // per configuration
+  const double *obs;       // per configuration
+  const double *beta_slot;
+  int32_t *config_of_slot;
+  int32_t *slot_of_config;
+  int32_t *boost_slot;
+  long long *pt_cnt, *pt_exc;
+  double *moments;         // per slot, NMOM
+  long long *n_meas;
+  double *etrace;          // per slot energies of this sweep (nullable)
+};
+
+__device__ double stag_mag(const double *sub, int n0, double iN) {
+  // absStaggMag (CF/subrouts.cpp:5382-5462)
+  const double norm = (double)n0 * iN;
+  double sx = 0, sy = 0, sz = 0, sq = 0;
+  for (int n = 0; n < n0; n++) {
+    const double mx = norm * sub[3 * n], my = norm * sub[3 * n + 1], mz = norm * sub[3 * n + 2];
+    sx += mx; sy += my; sz += mz;
+    sq += mx * mx + my * my + mz * mz;
+  }
+  const double r = sq - (sx * sx + sy * sy + sz * sz) / (double)n0;
+  return (1.0 / sqrt((double)n0)) * sqrt(fmax(r, 0.0));
+}
+
+__global__ void measure_exchange_kernel(const PTArgs a) {
+  const int t = threadIdx.x;
+  const double iN = 1.0 / (double)a.N;
+  if (t < a.R) {
+    const int k = t;
+    const int c = a.config_of_slot[k];
+    const double E = a.energy[c];
+    if (a.etrace) a.etrace[k] = E;
+    if (a.measure) {
+      const double *o = a.obs + (size_t)c * NOBS;
+      double *m = a.moments + (size_t)k * NMOM;
+      const double F1 = iN * sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+      const double F2 = F1 * F1;
+      double T1 = (a.ising ? 0.0 : o[2]) * iN;
+      if (!a.h_on) T1 = fabs(T1);
+      double Q1 = 0, SH = 0, SV = 0, SD = 0, X1 = 0;
+      if (a.has_labels) {
+        Q1 = stag_mag(o + 3, a.n_stag, iN);
+        SH = stag_mag(o + 12, 2, iN);
+        SV = stag_mag(o + 18, 2, iN);
+        SD = stag_mag(o + 24, 2, iN);
+        if (a.c3) {
+          // cplxC3Parameter (CF/subrouts.cpp:5730-5800), z = dMag component
+          const double s0 = o[5], s1 = o[8], s2 = o[11];
+          const double re = sqrt(6.0) * iN * (s0 + s1 - 2.0 * s2);
+          const double im = -3.0 * sqrt(2.0) * iN * (s1 - s0);
+          X1 = sqrt(re * re + im * im);
+        } else if (a.z2) {
+          X1 = iN * o[30];
+        }
+      }
+      const double S1 = sqrt(SH * SH + SV * SV + (a.c3 ? SD * SD : 0.0));
+      const double Q2 = Q1 * Q1, S2 = S1 * S1, X2 = X1 * X1;
+      m[0] += E; m[1] += E * E;
+      m[2] += F1; m[3] += F2; m[4] += F2 * F2;
+      m[5] += T1; m[6] += T1 * T1;
+      m[7] += Q1; m[8] += Q2; m[9] += Q2 * Q2;
+      m[10] += S1; m[11] += S2; m[12] += S2 * S2;
+      m[13] += SH; m[14] += SV; m[15] += SD;
+      m[16] += (a.z2 && !a.c3) ? fabs(X1) : X1; m[17] += X2; m[18] += X2 * X2;
+      if (k == 0) a.n_meas[0] += 1;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 0; k < a.R; k++) a.boost_slot[k] = 0;
+    if (a.parity >= 0) {
+      for (int k1 = a.parity; k1 + 1 < a.R; k1 += 2) {
+        const int k2 = k1 + 1;
+        a.pt_cnt[k1] += 1;
+        a.pt_cnt[k2] += 1;
+        const double ratio = (double)(a.pt_exc[k2] + 1) * (100.0 / (double)a.pt_cnt[k2]);
+        const int c1 = a.config_of_slot[k1], c2 = a.config_of_slot[k2];
+        const double dE = a.energy[c2] - a.energy[c1];
+        const double dB = a.beta_slot[k2] - a.beta_slot[k1];
+        const double fc = dB * dE;
+        const u32x4 r = draw4(a.seed, (uint32_t)k1, a.sweep, 0u, STREAM_PT);
+        const double u = u01_f64(r.x, r.y);
+        if (accept_exp(u, fc) && (!a.cap50 || ratio <= 50.0)) {
+          a.config_of_slot[k1] = c2;
+          a.config_of_slot[k2] = c1;
+          a.slot_of_config[c2] = k1;
+          a.slot_of_config[c1] = k2;
+          a.pt_exc[k1] += 1;
+          a.pt_exc[k2] += 1;
+          a.boost_slot[k1] = 1;
+          a.boost_slot[k2] = 1;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ engine
+
+struct MCEngine {
+  Lattice lat;
+  hsmd_mc_spec spec{};
+  int R = 0;
+  bool ising = false;
+  int prec = HSMD_BATCHED_F32;
+  bool use_lds = false;
+  size_t lds_bytes = 0;
+  uint32_t sweep = 0;
+  bool spins_set = false;
+  std::vector<double> temps;
+  DevBuf<int32_t> nb, perm, bi, bj, z2nb;
+  DevBuf<uint8_t> bc;
+  DevBuf<int8_t> lab;
+  DevBuf<float> Jf;
+  DevBuf<double> Jd;
+  DevBuf<unsigned char> spins;
+  DevBuf<double> beta, energy, obs, moments, etrace;
+  DevBuf<int32_t> cfg_of_slot, slot_of_cfg, boost;
+  DevBuf<long long> pt_cnt, pt_exc, n_meas;
+  SweepArgs args{};
+};
+
+static void fill_coup(const Lattice &L, Coup &cp, bool ising) {
+  cp.N = L.N;
+  cp.Npad = L.N + 1;
+  cp.Z = L.Z;
+  cp.general = L.general ? 1 : 0;
+  cp.h_on = L.h_on ? 1 : 0;
+  cp.h = L.c.h;
+  for (int s = 0; s < kZMAX; s++) { cp.jxy[s] = cp.jz[s] = 0.0; cp.lam[s] = 1.0; }
+  for (int s = 0; s < L.Z; s++) {
+    const int sh = L.slot_shell[s];
+    cp.jxy[s] = L.shellJ[sh - 1];
+    cp.jz[s] = ising ? L.shellJ[sh - 1] : L.shellJ[sh - 1] * L.shellLam[sh - 1];
+    cp.lam[s] = L.shellLam[sh - 1];
+  }
+}
+
+void upload_lattice_common(const Lattice &L, DevBuf<int32_t> &nb, DevBuf<int32_t> &perm,
+                           DevBuf<int32_t> &bi, DevBuf<int32_t> &bj, DevBuf<uint8_t> &bc,
+                           DevBuf<float> &Jf, DevBuf<double> &Jd, bool f32) {
+  nb.upload(L.g_nb.data(), L.g_nb.size());
+  perm.upload(L.g_perm.data(), L.g_perm.size());
+  bi.upload(L.g_bi.data(), L.g_bi.size());
+  bj.upload(L.g_bj.data(), L.g_bj.size());
+  bc.upload(L.g_bc.data(), L.g_bc.size());
+  if (L.general) {
+    if (f32) {
+      std::vector<float> t(L.g_J.begin(), L.g_J.end());
+      Jf.upload(t.data(), t.size());
+    } else {
+      Jd.upload(L.g_J.data(), L.g_J.size());
+    }
+  }
+}
+
+void fill_bonds(const Lattice &L, Bonds &bd) {
+  bd.Nb = (int)L.g_bi.size();
+  bd.n_class = (int)L.bc_Jxy.size();
+  for (int k = 0; k < kBCMAX; k++) { bd.jxy[k] = bd.jz[k] = 0.0; }
+  for (int k = 0; k < bd.n_class; k++) { bd.jxy[k] = L.bc_Jxy[k]; bd.jz[k] = L.bc_Jz[k]; }
+}
+
+static size_t lds_limit() { return 160 * 1024 - NOBS * 16 * 8 - (kBCMAX + 1) * 16 * 8 - 1024; }
+
+MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp) {
+  if (R < 1 || R > kThreads) throw Error(HSMD_ERR_ARG, "replica count must be in [1, 1024]");
+  if (sp.model != HSMD_HEISENBERG && sp.model != HSMD_ISING) throw Error(HSMD_ERR_ARG, "unknown model");
+  if (sp.precision < 0 || sp.precision > 2) throw Error(HSMD_ERR_ARG, "unknown precision form");
+  if (sp.n_or < 0 || sp.n_hb < 0 || sp.n_or_boost < 0 || sp.n_hb_boost < 0 ||
+      sp.n_or + sp.n_hb > 16 || sp.n_or_boost + sp.n_hb_boost > 16)
+    throw Error(HSMD_ERR_ARG, "passes per sweep must be in [0, 16]");
+  for (int k = 0; k < R; k++)
+    if (!(temps[k] > 0.0)) throw Error(HSMD_ERR_ARG, "temperatures must be positive");
+  if (L.n_colours > kCMAX) throw Error(HSMD_ERR_ARG, "too many colours");
+  std::unique_ptr<MCEngine> e(new MCEngine());
+  e->lat = L;
+  e->spec = sp;
+  e->R = R;
+  e->ising = sp.model == HSMD_ISING;
+  e->prec = sp.precision;
+  e->temps.assign(temps, temps + R);
+  const Lattice &lat = e->lat;
+  const bool f32 = (e->prec == HSMD_BATCHED_F32) && !e->ising;
+  upload_lattice_common(lat, e->nb, e->perm, e->bi, e->bj, e->bc, e->Jf, e->Jd, f32);
+  if (e->ising && lat.general) e->Jd.upload(lat.g_J.data(), lat.g_J.size());
+  // labels in new order
+  const int N = lat.N;
+  if (lat.labels.present) {
+    std::vector<int8_t> lb((size_t)4 * N);
+    for (int p = 0; p < N; p++) {
+      const int i = lat.perm[p];
+      lb[p] = lat.labels.stag[i];
+      lb[N + p] = lat.labels.hs[i];
+      lb[2 * N + p] = lat.labels.vs[i];
+      lb[3 * N + p] = lat.labels.ds[i];
+    }
+    e->lab.upload(lb.data(), lb.size());
+    if (lat.labels.z2) {
+      std::vector<int32_t> z((size_t)3 * N);
+      for (int p = 0; p < N; p++) {
+        const int i = lat.perm[p];
+        for (int r = 0; r < 3; r++) {
+          const int q = lat.labels.z2nb[(size_t)r * N + i];
+          z[(size_t)r * N + p] = q >= 0 ? lat.inv[q] : N;
+        }
+      }
+      e->z2nb.upload(z.data(), z.size());
+    }
+  }
+  const size_t Npad = (size_t)N + 1;
+  size_t elem = e->ising ? 1 : (f32 ? 3 * sizeof(float) : 3 * sizeof(double));
+  e->spins.alloc(Npad * elem * R);
+  e->spins.zero();
+  e->lds_bytes = Npad * elem;
+  e->use_lds = (e->prec != HSMD_PLAIN_F64) && e->lds_bytes <= lds_limit();
+  std::vector<double> b(R);
+  for (int k = 0; k < R; k++) b[k] = 1.0 / temps[k];
+  e->beta.upload(b.data(), R);
+  e->energy.alloc(R); e->energy.zero();
+  e->obs.alloc((size_t)R * NOBS); e->obs.zero();
+  e->moments.alloc((size_t)R * NMOM); e->moments.zero();
+  std::vector<int32_t> id(R);
+  for (int k = 0; k < R; k++) id[k] = k;
+  e->cfg_of_slot.upload(id.data(), R);
+  e->slot_of_cfg.upload(id.data(), R);
+  e->boost.alloc(R); e->boost.zero();
+  e->pt_cnt.alloc(R); e->pt_cnt.zero();
+  e->pt_exc.alloc(R); e->pt_exc.zero();
+  e->n_meas.alloc(1); e->n_meas.zero();
+  // kernel arguments
+  SweepArgs &a = e->args;
+  fill_coup(lat, a.cp, e->ising);
+  fill_bonds(lat, a.bd);
+  a.n_colours = lat.n_colours;
+  for (int k = 0; k <= lat.n_colours; k++) a.col_off[k] = lat.col_off[k];
+  a.n_or = sp.n_or; a.n_hb = sp.n_hb; a.n_or_b = sp.n_or_boost; a.n_hb_b = sp.n_hb_boost;
+  a.seed = sp.seed;
+  a.n_stag = lat.labels.n_stag;
+  a.has_labels = lat.labels.present ? 1 : 0;
+  a.z2 = lat.labels.z2 ? 1 : 0;
+  a.c3 = lat.labels.c3 ? 1 : 0;
+  a.nb = e->nb.p;
+  a.Jt = f32 ? (const void *)e->Jf.p : (const void *)e->Jd.p;
+  a.perm = e->perm.p;
+  a.bi = e->bi.p; a.bj = e->bj.p; a.bc = e->bc.p;
+  a.lab = e->lab.p;
+  a.z2nb = e->z2nb.p;
+  a.spins = e->spins.p;
+  a.beta_slot = e->beta.p;
+  a.slot_of_config = e->slot_of_cfg.p;
+  a.boost_slot = e->boost.p;
+  a.energy = e->energy.p;
+  a.obs = e->obs.p;
+  if (e->use_lds) {
+    const void *fn;
+    if (e->ising) fn = (const void *)ising_sweep_kernel<true>;
+    else if (f32) fn = (const void *)heis_sweep_kernel<float, true>;
+    else fn = (const void *)heis_sweep_kernel<double, true>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_bytes));
+  }
+  return e.release();
+}
+
+void mc_set_spins(MCEngine &e, const double *spins) {
+  const Lattice &L = e.lat;
+  const int N = L.N, R = e.R;
+  const size_t Npad = (size_t)N + 1;
+  const bool f32 = e.prec == HSMD_BATCHED_F32;
+  std::vector<int32_t> id(R);
+  for (int k = 0; k < R; k++) id[k] = k;
+  HIP_CHECK(hipMemcpy(e.cfg_of_slot.p, id.data(), R * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(e.slot_of_cfg.p, id.data(), R * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (e.ising) {
+    std::vector<int8_t> h(Npad * R, 0);
+    for (int r = 0; r < R; r++)
+      for (int i = 0; i < N; i++)
+        h[r * Npad + L.inv[i]] = spins[((size_t)r * N + i) * 3] < 0.0 ? -1 : 1;
+    HIP_CHECK(hipMemcpy(e.spins.p, h.data(), h.size(), hipMemcpyHostToDevice));
+  } else if (f32) {
+    std::vector<float> h(3 * Npad * R, 0.f);
+    for (int r = 0; r < R; r++)
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++)
+          h[(r * 3 + d) * Npad + L.inv[i]] = (float)spins[((size_t)r * N + i) * 3 + d];
+    HIP_CHECK(hipMemcpy(e.spins.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> h(3 * Npad * R, 0.0);
+    for (int r = 0; r < R; r++)
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++)
+          h[(r * 3 + d) * Npad + L.inv[i]] = spins[((size_t)r * N + i) * 3 + d];
+    HIP_CHECK(hipMemcpy(e.spins.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  e.spins_set = true;
+}
+
+void mc_get_spins(MCEngine &e, double *spins) {
+  const Lattice &L = e.lat;
+  const int N = L.N, R = e.R;
+  const size_t Npad = (size_t)N + 1;
+  const bool f32 = e.prec == HSMD_BATCHED_F32;
+  std::vector<int32_t> cos(R);
+  HIP_CHECK(hipMemcpy(cos.data(), e.cfg_of_slot.p, R * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (e.ising) {
+    std::vector<int8_t> h(Npad * R);
+    HIP_CHECK(hipMemcpy(h.data(), e.spins.p, h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < R; k++)
+      for (int i = 0; i < N; i++) {
+        double *o = spins + ((size_t)k * N + i) * 3;
+        o[0] = (double)h[cos[k] * Npad + L.inv[i]];
+        o[1] = o[2] = 0.0;
+      }
+  } else if (f32) {
+    std::vector<float> h(3 * Npad * R);
+    HIP_CHECK(hipMemcpy(h.data(), e.spins.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int k = 0; k < R; k++)
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++)
+          spins[((size_t)k * N + i) * 3 + d] = h[(cos[k] * 3 + d) * Npad + L.inv[i]];
+  } else {
+    std::vector<double> h(3 * Npad * R);
+    HIP_CHECK(hipMemcpy(h.data(), e.spins.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < R; k++)
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++)
+          spins[((size_t)k * N + i) * 3 + d] = h[(cos[k] * 3 + d) * Npad + L.inv[i]];
+  }
+}
+
+static void launch_sweep(MCEngine &e, hipStream_t st) {
+  SweepArgs &a = e.args;
+  const bool f32 = e.prec == HSMD_BATCHED_F32;
+  if (e.prec == HSMD_PLAIN_F64) {
+    // one instance per launch, FP64, spins in global memory
+    for (int c = 0; c < e.R; c++) {
+      a.config_base = c;
+      if (e.ising) hipLaunchKernelGGL((ising_sweep_kernel<false>), dim3(1), dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((heis_sweep_kernel<double, false>), dim3(1), dim3(kThreads), 0, st, a);
+    }
+    a.config_base = 0;
+    return;
+  }
+  a.config_base = 0;
+  const size_t sh = e.use_lds ? e.lds_bytes : 0;
+  if (e.ising) {
+    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((ising_sweep_kernel<false>), dim3(e.R), dim3(kThreads), 0, st, a);
+  } else if (f32) {
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<float, false>), dim3(e.R), dim3(kThreads), 0, st, a);
+  } else {
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<double, true>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<double, false>), dim3(e.R), dim3(kThreads), 0, st, a);
+  }
+}
+
+void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrace_out) {
+  if (!e.spins_set) throw Error(HSMD_ERR_STATE, "hsmd_mc_run before hsmd_mc_set_spins");
+  if (n_sweeps < 0) throw Error(HSMD_ERR_ARG, "n_sweeps must be >= 0");
+  if (pt_parity < -1 || pt_parity > 2) throw Error(HSMD_ERR_ARG, "pt_parity must be -1, 0, 1 or 2");
+  if (etrace_out && n_sweeps > 0) e.etrace.alloc((size_t)n_sweeps * e.R);
+  PTArgs p{};
+  p.R = e.R; p.N = e.lat.N;
+  p.cap50 = e.spec.pt_cap50;
+  p.measure = measure;
+  p.n_stag = e.args.n_stag; p.has_labels = e.args.has_labels; p.z2 = e.args.z2; p.c3 = e.args.c3;
+  p.h_on = e.lat.h_on ? 1 : 0;
+  p.ising = e.ising ? 1 : 0;
+  p.seed = e.spec.seed;
+  p.energy = e.energy.p; p.obs = e.obs.p; p.beta_slot = e.beta.p;
+  p.config_of_slot = e.cfg_of_slot.p; p.slot_of_config = e.slot_of_cfg.p;
+  p.boost_slot = e.boost.p; p.pt_cnt = e.pt_cnt.p; p.pt_exc = e.pt_exc.p;
+  p.moments = e.moments.p; p.n_meas = e.n_meas.p;
+  e.args.measure = measure;
+  const int pthreads = ((e.R + 63) / 64) * 64;
+  for (int s = 0; s < n_sweeps; s++) {
+    e.args.sweep = e.sweep;
+    launch_sweep(e, 0);
+    p.sweep = e.sweep;
+    p.parity = pt_parity == 2 ? (int)(e.sweep % 2u) : pt_parity;
+    p.etrace = etrace_out ? e.etrace.p + (size_t)s * e.R : nullptr;
+    hipLaunchKernelGGL(measure_exchange_kernel, dim3(1), dim3(pthreads), 0, 0, p);
+    e.sweep++;
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  if (etrace_out && n_sweeps > 0)
+    HIP_CHECK(hipMemcpy(etrace_out, e.etrace.p, (size_t)n_sweeps * e.R * sizeof(double),
+                        hipMemcpyDeviceToHost));
+}
+
+void mc_energies(MCEngine &e, double *E) {
+  std::vector<double> ec(e.R);
+  std::vector<int32_t> cos(e.R);
+  HIP_CHECK(hipMemcpy(ec.data(), e.energy.p, e.R * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(cos.data(), e.cfg_of_slot.p, e.R * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int k = 0; k < e.R; k++) E[k] = ec[cos[k]];
+}
+
+void mc_refresh_energies(MCEngine &e) {
+  // zero-pass sweep: recompute energies of the current configurations
+  SweepArgs a = e.args;
+  a.n_or = a.n_hb = a.n_or_b = a.n_hb_b = 0;
+  a.measure = 0;
+  std::swap(a, e.args);
+  try {
+    launch_sweep(e, 0);
+  } catch (...) {
+    std::swap(a, e.args);
+    throw;
+  }
+  std::swap(a, e.args);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+void mc_pt_counters(MCEngine &e, int64_t *att, int64_t *acc) {
+  std::vector<long long> a(e.R), b(e.R);
+  HIP_CHECK(hipMemcpy(a.data(), e.pt_cnt.p, e.R * sizeof(long long), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(b.data(), e.pt_exc.p, e.R * sizeof(long long), hipMemcpyDeviceToHost));
+  for (int k = 0; k < e.R; k++) { att[k] = a[k]; acc[k] = b[k]; }
+}
+
+void mc_reset_counters(MCEngine &e) {
+  e.pt_cnt.zero();
+  e.pt_exc.zero();
+  e.boost.zero();
+  e.sweep = 0;
+}
+
+void mc_moments(MCEngine &e, double *out, int64_t *n, int reset) {
+  HIP_CHECK(hipMemcpy(out, e.moments.p, (size_t)e.R * NMOM * sizeof(double), hipMemcpyDeviceToHost));
+  long long nm = 0;
+  HIP_CHECK(hipMemcpy(&nm, e.n_meas.p, sizeof(long long), hipMemcpyDeviceToHost));
+  if (n) *n = nm;
+  if (reset) { e.moments.zero(); e.n_meas.zero(); }
+}
+
+void mc_slot_map(MCEngine &e, int32_t *m) {
+  HIP_CHECK(hipMemcpy(m, e.cfg_of_slot.p, e.R * sizeof(int32_t), hipMemcpyDeviceToHost));
+}
+
+void mc_destroy(MCEngine *e) { delete e; }
+
+// Sweep-kernel timing hook for the benchmark: launches n sweeps of every
+// replica back to back on stream st (no exchange), without synchronising.
+void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st) {
+  for (int s = 0; s < n; s++) {
+    e.args.sweep = e.sweep++;
+    e.args.measure = 0;
+    launch_sweep(e, st);
+  }
+}
+
+// ------------------------------------------------- fields / energies (ABI)
+
+template <typename T>
+__global__ void fields_kernel(const Coup cp, const int32_t *nb, const T *Jt, const T *spins,
+                              T *out) {
+  const int c = blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cp.N) return;
+  const T *s = spins + (size_t)c * 3 * cp.Npad;
+  T hx, hy, hz;
+  local_field<T>(cp, nb, Jt, p, s, s + cp.Npad, s + 2 * cp.Npad, hx, hy, hz);
+  T *o = out + (size_t)c * 3 * cp.N;
+  o[p] = hx; o[cp.N + p] = hy; o[2 * cp.N + p] = hz;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void energy_kernel(const Coup cp, const Bonds bd,
+                                                          const int32_t *bi, const int32_t *bj,
+                                                          const uint8_t *bc, const T *spins,
+                                                          double *E) {
+  __shared__ double scratch[16];
+  const int c = blockIdx.x;
+  const T *s = spins + (size_t)c * 3 * cp.Npad;
+  const T *sx = s, *sy = s + cp.Npad, *sz = s + 2 * cp.Npad;
+  double e = 0.0, mz = 0.0;
+  for (int b = threadIdx.x; b < bd.Nb; b += blockDim.x) {
+    const int i = bi[b], j = bj[b], k = bc[b];
+    e += bd.jxy[k] * ((double)sx[i] * (double)sx[j] + (double)sy[i] * (double)sy[j]) +
+         bd.jz[k] * ((double)sz[i] * (double)sz[j]);
+  }
+  for (int p = threadIdx.x; p < cp.N; p += blockDim.x) mz += (double)sz[p];
+  double v[1] = {e - (cp.h_on ? cp.h * mz : 0.0)};
+  block_sum<1>(v, scratch);
+  if (threadIdx.x == 0) E[c] = v[0];
+}
+
+template <typename T>
+static void fields_energies_impl(const Lattice &L, int R, const double *spins, double *fields,
+                                 double *energies) {
+  const int N = L.N;
+  const size_t Npad = (size_t)N + 1;
+  DevBuf<int32_t> nb, perm, bi, bj;
+  DevBuf<uint8_t> bc;
+  DevBuf<float> Jf;
+  DevBuf<double> Jd;
+  const bool f32 = sizeof(T) == 4;
+  upload_lattice_common(L, nb, perm, bi, bj, bc, Jf, Jd, f32);
+  Coup cp;
+  fill_coup(L, cp, false);
+  Bonds bd;
+  fill_bonds(L, bd);
+  std::vector<T> h(3 * Npad * R, T(0));
+  for (int r = 0; r < R; r++)
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++) h[(r * 3 + d) * Npad + L.inv[i]] = (T)spins[((size_t)r * N + i) * 3 + d];
+  DevBuf<T> ds;
+  ds.upload(h.data(), h.size());
+  const T *Jt = f32 ? (const T *)Jf.p : (const T *)Jd.p;
+  if (fields) {
+    DevBuf<T> out;
+    out.alloc((size_t)3 * N * R);
+    hipLaunchKernelGGL(fields_kernel<T>, dim3((N + 255) / 256, R), dim3(256), 0, 0, cp, nb.p, Jt,
+                       ds.p, out.p);
+    HIP_CHECK(hipGetLastError());
+    std::vector<T> o((size_t)3 * N * R);
+    HIP_CHECK(hipMemcpy(o.data(), out.p, o.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; r++)
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++)
+          fields[((size_t)r * N + i) * 3 + d] = (double)o[((size_t)r * 3 + d) * N + L.inv[i]];
+  }
+  if (energies) {
+    DevBuf<double> E;
+    E.alloc(R);
+    hipLaunchKernelGGL(energy_kernel<T>, dim3(R), dim3(kThreads), 0, 0, cp, bd, bi.p, bj.p, bc.p,
+                       ds.p, E.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(energies, E.p, R * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
+                     double *energies) {
+  if (R < 1) throw Error(HSMD_ERR_ARG, "R must be >= 1");
+  if (precision == HSMD_BATCHED_F32) fields_energies_impl<float>(L, R, spins, fields, energies);
+  else fields_energies_impl<double>(L, R, spins, fields, energies);
+}
+
+// ---------------------------------------------------------- self checks
+
+__global__ void philox_kernel(uint64_t seed, uint32_t step, uint32_t inst, uint32_t stream, int n,
+                              uint32_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32x4 r = draw4(seed, (uint32_t)i, step, inst, stream);
+  out[4 * i] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = r.w;
+}
+
+__global__ void exp_kernel(int n, const double *x, double *y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = det_exp_neg(x[i]);
+}
+
+void debug_philox(uint64_t seed, uint32_t step, uint32_t inst, uint32_t stream, int n, uint32_t *out) {
+  if (n <= 0) return;
+  DevBuf<uint32_t> d;
+  d.alloc((size_t)4 * n);
+  hipLaunchKernelGGL(philox_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, seed, step, inst, stream, n, d.p);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(out, d.p, (size_t)4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+}
+
+void debug_exp(int n, const double *x, double *y) {
+  if (n <= 0) return;
+  DevBuf<double> dx, dy;
+  dx.upload(x, n);
+  dy.alloc(n);
+  hipLaunchKernelGGL(exp_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, dx.p, dy.p);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(y, dy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+}  // namespace hsmd

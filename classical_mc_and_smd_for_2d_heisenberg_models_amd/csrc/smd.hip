@@ -1,0 +1,779 @@
+// Semiclassical spin dynamics (SMD) engine: RK4 on the undamped LLG
+// equations, the energy-correction step, and DSSF accumulation.
+//
+// Reference: get_dynSpectrum CF/subrouts.cpp:11547-12149, driven by the RK
+// loop CF/smd_code.cpp:3079-3312 (one MPI rank per slice of samples) and the
+// root reduction CF/smd_code.cpp:3330-3396.
+//
+// Per time step and sample (reference order):
+//   K1..K4 = dt (h(S_k) x S_k) with S_1 = S, S_2 = S + K1/2, S_3 = S + K2/2,
+//   S_4 = S + K3 (stages unnormalised), S <- norm(S + (K1+K4)/6 + (K2+K3)/3);
+//   c = (E_ref - E(S)) / sum_i |h_i x S_i|^2, S_i <- norm(S_i + c S_i x (h_i x S_i));
+//   qSS_t = (1/N) sum_i S_i(0).S_i(t);  S(q,t) = DFT2D(S)/sqrt(Nq).
+// After the loop: X(q,w) = DFT_t(w(t) S(q,t))/sqrt(T), DSSF = |X|^2.
+//
+// Design: one workgroup per sample, all samples (of any mix of lattices) in
+// one launch that integrates a chunk of time steps with the lattice resident
+// in LDS; FP64 block reductions for E_ref, E(S), sum |h x S|^2 and qSS; the
+// correction is applied in FP64 (Jacobi order: every site uses the fields of
+// the post-RK configuration).  Each step's configuration is written to a
+// trajectory chunk; the DSSF is linear in the spins, so the time transform is
+// contracted first, Y(r,w) += sum_t A(t, r) B(t, w) with B = window x
+// twiddles, as a GEMM per chunk, and the spatial transform of Y and |.|^2
+// follow once at the end.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <memory>
+
+#include "device_common.h"
+#include "hsmd_internal.h"
+
+namespace hsmd {
+
+struct LatDev {
+  Coup cp;
+  Bonds bd;
+  const int32_t *nb;
+  const void *Jt;
+  const int32_t *bi, *bj;
+  const uint8_t *bc;
+};
+
+struct SampleDev {
+  int li;
+  int N;
+  long long spin_off;   // into spins / S0 (3 * Npad per sample)
+  long long traj_off;   // into trajectory chunk (Tc * 3 * N per sample)
+  long long y_off;      // into Y accumulators (3 * N * 2 * n_omega per sample)
+};
+
+constexpr int kSmdThreads = 1024;
+
+template <typename T>
+__device__ __forceinline__ void cross3(T ax, T ay, T az, T bx, T by, T bz, T &cx, T &cy, T &cz) {
+  cx = ay * bz - az * by;
+  cy = az * bx - ax * bz;
+  cz = ax * by - ay * bx;
+}
+
+template <typename T>
+__device__ __forceinline__ double bond_energy(const LatDev &L, const T *sx, const T *sy, const T *sz) {
+  double e = 0.0;
+  for (int b = threadIdx.x; b < L.bd.Nb; b += blockDim.x) {
+    const int i = L.bi[b], j = L.bj[b], k = L.bc[b];
+    e += L.bd.jxy[k] * ((double)sx[i] * (double)sx[j] + (double)sy[i] * (double)sy[j]) +
+         L.bd.jz[k] * ((double)sz[i] * (double)sz[j]);
+  }
+  return e;
+}
+
+template <typename T>
+__device__ __forceinline__ double field_energy_term(const LatDev &L, const T *sz) {
+  if (!L.cp.h_on) return 0.0;
+  double m = 0.0;
+  for (int p = threadIdx.x; p < L.cp.N; p += blockDim.x) m += (double)sz[p];
+  return -L.cp.h * m;
+}
+
+struct StepArgs {
+  const LatDev *lats;
+  const SampleDev *samples;
+  void *spins;          // per sample [3][Npad]
+  const void *s0;       // per sample [3][Npad], t = 0 spins
+  void *traj;           // per sample [Tc][3][N]
+  double *qss;          // per sample [T]
+  double *eref;         // per sample, energy at the current time
+  void *stage;          // per sample [3][Npad] stage buffer (global-memory form)
+  double dt;
+  int t0;               // global time index of the first point of this launch
+  int n_points;         // time points produced by this launch
+  int chunk_base;       // time index stored at trajectory row 0
+  int T;                // total time points
+  int lds_npad;         // LDS stride (max Npad) when LDS is used
+  int sample_base;      // first sample of this launch
+};
+
+// One launch produces n_points time points t0 .. t0+n_points-1 of every
+// sample; t = 0 is the input configuration, every later point is one RK4
+// step plus the energy correction.
+template <typename T, int SPT, bool LDS>
+__global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[2 * 16];
+  const int sidx = blockIdx.x + a.sample_base;
+  const SampleDev sd = a.samples[sidx];
+  const LatDev &L = a.lats[sd.li];
+  const int N = sd.N, Npad = N + 1;
+  T *gS = (T *)a.spins + sd.spin_off;
+  const T *g0 = (const T *)a.s0 + sd.spin_off;
+  T *S, *B;
+  int stride;
+  if (LDS) {
+    stride = a.lds_npad;
+    S = (T *)smem;
+    B = S + 3 * stride;
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) {
+      const int d = i / Npad, p = i - d * Npad;
+      S[d * stride + p] = gS[i];
+      B[d * stride + p] = T(0);
+    }
+  } else {
+    // global-memory form: spins in place, stage buffer beside them
+    stride = Npad;
+    S = gS;
+    B = (T *)a.stage + sd.spin_off;
+  }
+  __syncthreads();
+  T *Sx = S, *Sy = S + stride, *Sz = S + 2 * stride;
+  T *Bx = B, *By = B + stride, *Bz = B + 2 * stride;
+  const T dt = T(a.dt);
+  double Eref = a.eref[sidx];
+  const T *Jt = (const T *)L.Jt;
+  T *traj = (T *)a.traj + sd.traj_off;
+  for (int it = 0; it < a.n_points; it++) {
+    const int t = a.t0 + it;
+    if (t > 0) {
+      T K1[SPT][3], K2[SPT][3], K3[SPT][3];
+      // stage 1: K1 from S, B = S + K1/2
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          T hx, hy, hz;
+          local_field<T>(L.cp, L.nb, Jt, p, Sx, Sy, Sz, hx, hy, hz);
+          const T x = Sx[p], y = Sy[p], z = Sz[p];
+          T px, py, pz;
+          cross3(hx, hy, hz, x, y, z, px, py, pz);
+          K1[u][0] = dt * px; K1[u][1] = dt * py; K1[u][2] = dt * pz;
+          Bx[p] = x + T(0.5) * K1[u][0];
+          By[p] = y + T(0.5) * K1[u][1];
+          Bz[p] = z + T(0.5) * K1[u][2];
+        }
+      }
+      __syncthreads();
+      // stage 2
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          T hx, hy, hz;
+          local_field<T>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          T px, py, pz;
+          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
+          K2[u][0] = dt * px; K2[u][1] = dt * py; K2[u][2] = dt * pz;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          Bx[p] = Sx[p] + T(0.5) * K2[u][0];
+          By[p] = Sy[p] + T(0.5) * K2[u][1];
+          Bz[p] = Sz[p] + T(0.5) * K2[u][2];
+        }
+      }
+      __syncthreads();
+      // stage 3
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          T hx, hy, hz;
+          local_field<T>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          T px, py, pz;
+          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
+          K3[u][0] = dt * px; K3[u][1] = dt * py; K3[u][2] = dt * pz;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          Bx[p] = Sx[p] + K3[u][0];
+          By[p] = Sy[p] + K3[u][1];
+          Bz[p] = Sz[p] + K3[u][2];
+        }
+      }
+      __syncthreads();
+      // stage 4 and combine (S is not read by any thread in this stage)
+      T Sn[SPT][3];
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          T hx, hy, hz;
+          local_field<T>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          T px, py, pz;
+          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
+          const T f1 = T(1.0 / 6.0), f2 = T(2.0 / 6.0);
+          T v[3];
+          const T k4[3] = {dt * px, dt * py, dt * pz};
+          const T s[3] = {Sx[p], Sy[p], Sz[p]};
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+            v[d] = s[d] + (f1 * (K1[u][d] + k4[d]) + f2 * (K2[u][d] + K3[u][d]));
+          const T inrm = T(1) / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          Sn[u][0] = v[0] * inrm; Sn[u][1] = v[1] * inrm; Sn[u][2] = v[2] * inrm;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) { Sx[p] = Sn[u][0]; Sy[p] = Sn[u][1]; Sz[p] = Sn[u][2]; }
+      }
+      __syncthreads();
+      // energy correction: sum |h x S|^2 and E(S) in FP64
+      double D[SPT][3];
+      double v2[2] = {0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          T hx, hy, hz;
+          local_field<T>(L.cp, L.nb, Jt, p, Sx, Sy, Sz, hx, hy, hz);
+          const double x = Sx[p], y = Sy[p], z = Sz[p];
+          double px, py, pz;
+          cross3<double>(hx, hy, hz, x, y, z, px, py, pz);
+          v2[0] += px * px + py * py + pz * pz;
+          cross3<double>(x, y, z, px, py, pz, D[u][0], D[u][1], D[u][2]);
+        }
+      }
+      v2[1] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
+      block_sum<2>(v2, scratch);
+      const double cfac = (Eref - v2[1]) / v2[0];
+      double v3[2] = {0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = threadIdx.x + u * kSmdThreads;
+        if (p < N) {
+          double x = (double)Sx[p] + cfac * D[u][0];
+          double y = (double)Sy[p] + cfac * D[u][1];
+          double z = (double)Sz[p] + cfac * D[u][2];
+          const double nr = sqrt(x * x + y * y + z * z);
+          x /= nr; y /= nr; z /= nr;
+          v3[1] += (double)g0[p] * x + (double)g0[Npad + p] * y + (double)g0[2 * Npad + p] * z;
+          Sx[p] = (T)x; Sy[p] = (T)y; Sz[p] = (T)z;
+        }
+      }
+      __syncthreads();
+      v3[0] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
+      block_sum<2>(v3, scratch);
+      Eref = v3[0];
+      if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t] = v3[1] / (double)N;
+    } else {
+      if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + 0] = 1.0;
+    }
+    // record the configuration at time t
+    if (a.traj) {
+      T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
+      for (int p = threadIdx.x; p < N; p += blockDim.x) {
+        row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
+      }
+    }
+  }
+  if (threadIdx.x == 0) a.eref[sidx] = Eref;
+  if (LDS) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) {
+      const int d = i / Npad, p = i - d * Npad;
+      gS[i] = S[d * stride + p];
+    }
+  }
+}
+
+// Energy of every sample (FP64 reduction), used to seed E_ref.
+template <typename T>
+__global__ __launch_bounds__(kSmdThreads) void smd_energy_kernel(const LatDev *lats,
+                                                                 const SampleDev *samples,
+                                                                 const T *spins, double *E) {
+  __shared__ double scratch[16];
+  const SampleDev sd = samples[blockIdx.x];
+  const LatDev &L = lats[sd.li];
+  const int Npad = sd.N + 1;
+  const T *s = spins + sd.spin_off;
+  double v[1] = {bond_energy<T>(L, s, s + Npad, s + 2 * Npad) + field_energy_term<T>(L, s + 2 * Npad)};
+  block_sum<1>(v, scratch);
+  if (threadIdx.x == 0) E[blockIdx.x] = v[0];
+}
+
+// ------------------------------------------------------------- DSSF GEMM
+
+// Y[s][m][j] += sum_k A[s][k][m] * B[k0+k][j], m < 3 N_s, j < 2 n_omega.
+// 64 x 64 output tile per workgroup of 256 threads, 4 x 4 per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void time_gemm_kernel(const SampleDev *samples, const T *A,
+                                                        const T *Bm, int ldb, int k0, int K,
+                                                        T *Y) {
+  __shared__ T As[16][64];
+  __shared__ T Bs[16][64];
+  const SampleDev sd = samples[blockIdx.z];
+  const int M = 3 * sd.N;
+  const int m0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  if (m0 >= M) return;
+  const T *Ab = A + sd.traj_off;
+  T *Yb = Y + sd.y_off;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = T(0);
+  for (int kk = 0; kk < K; kk += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int r = e >> 6, cidx = e & 63;
+      const int k = kk + r;
+      const int m = m0 + cidx;
+      As[r][cidx] = (k < K && m < M) ? Ab[(size_t)k * M + m] : T(0);
+      Bs[r][cidx] = (k < K) ? Bm[(size_t)(k0 + k) * ldb + j0 + cidx] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[r][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[r][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int m = m0 + ty * 4 + i;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; j++) Yb[(size_t)m * ldb + j0 + tx * 4 + j] += acc[i][j];
+  }
+}
+
+// --------------------------------------------------- spatial transform
+
+// Row pass over the DFT grid of one sample: for grid row a and component d,
+// R[a][b'][n] = sum_b Yg[a][b][n] e^{-2 pi i b b'/G}, Yg the site values
+// scattered onto the grid (0 where no site).  One workgroup per (row, d, s).
+template <typename T>
+__global__ void dft_rows_kernel(const SampleDev *samples, const int *sample_ids,
+                                const int32_t *gmap, int G, int n_omega, const T *Y,
+                                const T *cs, const T *sn, T *Rout) {
+  const int a = blockIdx.x, d = blockIdx.y, sl = blockIdx.z;
+  const SampleDev sd = samples[sample_ids[sl]];
+  const int N = sd.N;
+  const int ldy = 2 * n_omega;
+  const T *Yb = Y + sd.y_off;
+  T *Rb = Rout + ((size_t)(sl * 3 + d) * G + a) * (size_t)G * ldy;
+  for (int e = threadIdx.x; e < G * n_omega; e += blockDim.x) {
+    const int bp = e / n_omega, n = e - bp * n_omega;
+    T re = T(0), im = T(0);
+    for (int b = 0; b < G; b++) {
+      const int site = gmap[a * G + b];
+      if (site < 0) continue;
+      const T *y = Yb + (size_t)(d * N + site) * ldy;
+      const T yr = y[n], yi = y[n_omega + n];
+      const int ph = (b * bp) % G;
+      const T c = cs[ph], s = sn[ph];   // e^{-i th} = c - i s
+      re += yr * c + yi * s;
+      im += yi * c - yr * s;
+    }
+    T *o = Rb + (size_t)bp * ldy;
+    o[n] = re;
+    o[n_omega + n] = im;
+  }
+}
+
+// Column pass: X[a'][b'][n] = sum_a R[a][b'][n] e^{-2 pi i a a'/G}; the
+// squared moduli, scaled by 1/(Nq T), are summed over the samples of the
+// batch into dssf[d][q = a' G + b'][n].
+template <typename T>
+__global__ void dft_cols_kernel(int n_batch, int G, int n_omega, const T *R, const T *cs,
+                                const T *sn, double scale, double *dssf) {
+  const int bp = blockIdx.x, d = blockIdx.y;
+  const int ldy = 2 * n_omega;
+  const int Nq = G * G;
+  for (int e = threadIdx.x; e < G * n_omega; e += blockDim.x) {
+    const int ap = e / n_omega, n = e - ap * n_omega;
+    double acc = 0.0;
+    for (int sl = 0; sl < n_batch; sl++) {
+      const T *Rb = R + (size_t)(sl * 3 + d) * G * (size_t)G * ldy;
+      T re = T(0), im = T(0);
+      for (int a = 0; a < G; a++) {
+        const T *r = Rb + ((size_t)a * G + bp) * ldy;
+        const T rr = r[n], ri = r[n_omega + n];
+        const int ph = (a * ap) % G;
+        const T c = cs[ph], s = sn[ph];
+        re += rr * c + ri * s;
+        im += ri * c - rr * s;
+      }
+      acc += (double)re * (double)re + (double)im * (double)im;
+    }
+    dssf[((size_t)d * Nq + (size_t)ap * G + bp) * n_omega + n] += scale * acc;
+  }
+}
+
+// ---------------------------------------------------------------- engine
+
+struct SMDEngine {
+  std::vector<Lattice> lats;
+  hsmd_smd_spec spec{};
+  int S = 0;
+  bool f32 = false;
+  bool use_lds = false;
+  int spt = 1;
+  size_t lds_bytes = 0;
+  int lds_npad = 0;
+  int Tc = 0;
+  int t_next = 0;        // next time index to produce
+  int chunk_base = 0;
+  int chunk_fill = 0;
+  bool dssf_done = false;
+  std::vector<int32_t> sample_lat;
+  std::vector<SampleDev> hs;
+  std::vector<double> window;
+  // device
+  std::vector<DevBuf<int32_t>> nb, bi, bj, gmap;
+  std::vector<DevBuf<uint8_t>> bc;
+  std::vector<DevBuf<float>> Jf;
+  std::vector<DevBuf<double>> Jd;
+  DevBuf<LatDev> dlats;
+  DevBuf<SampleDev> dsamples;
+  DevBuf<unsigned char> spins, s0, stage, traj, ybuf, twid, dftcs;
+  DevBuf<double> qss, eref, dssf;
+  DevBuf<int> sample_ids;
+  bool set = false;
+};
+
+template <typename T>
+static void upload_twiddles(SMDEngine &e) {
+  const int T_ = e.spec.n_steps, no = e.spec.n_omega;
+  std::vector<T> B((size_t)T_ * 2 * no);
+  const double tp = 2.0 * acos(-1.0);
+  for (int t = 0; t < T_; t++) {
+    const double w = e.window[t];
+    for (int n = 0; n < no; n++) {
+      const long long ph = ((long long)n * t) % T_;
+      const double th = tp * (double)ph / (double)T_;
+      B[(size_t)t * 2 * no + n] = (T)(w * cos(th));
+      B[(size_t)t * 2 * no + no + n] = (T)(-w * sin(th));
+    }
+  }
+  e.twid.alloc(B.size() * sizeof(T));
+  HIP_CHECK(hipMemcpy(e.twid.p, B.data(), B.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
+                      const hsmd_smd_spec &sp, const double *window) {
+  if (S < 1) throw Error(HSMD_ERR_ARG, "sample count must be >= 1");
+  if (!(sp.dt > 0.0)) throw Error(HSMD_ERR_ARG, "dt must be positive");
+  if (sp.n_steps < 2) throw Error(HSMD_ERR_ARG, "n_steps must be >= 2");
+  if (sp.precision < 0 || sp.precision > 2) throw Error(HSMD_ERR_ARG, "unknown precision form");
+  if (sp.gauss_seidel) throw Error(HSMD_ERR_ARG, "in-place (Gauss-Seidel) correction is CPU-only");
+  if (sp.dssf && (sp.n_omega < 1 || sp.n_omega > sp.n_steps))
+    throw Error(HSMD_ERR_ARG, "n_omega must be in [1, n_steps]");
+  std::unique_ptr<SMDEngine> e(new SMDEngine());
+  e->spec = sp;
+  e->S = S;
+  e->f32 = sp.precision == HSMD_BATCHED_F32;
+  const size_t es = e->f32 ? sizeof(float) : sizeof(double);
+  for (int i = 0; i < nl; i++) e->lats.push_back(*lats[i]);
+  e->sample_lat.assign(sample_lattice, sample_lattice + S);
+  int maxN = 0;
+  for (int s = 0; s < S; s++) {
+    if (sample_lattice[s] < 0 || sample_lattice[s] >= nl) throw Error(HSMD_ERR_ARG, "bad sample lattice index");
+    maxN = std::max(maxN, e->lats[sample_lattice[s]].N);
+  }
+  e->window.assign(sp.n_steps, 1.0);
+  if (window) e->window.assign(window, window + sp.n_steps);
+  // per-lattice device tables
+  e->nb.resize(nl); e->bi.resize(nl); e->bj.resize(nl); e->bc.resize(nl);
+  e->Jf.resize(nl); e->Jd.resize(nl); e->gmap.resize(nl);
+  std::vector<LatDev> hl(nl);
+  for (int i = 0; i < nl; i++) {
+    const Lattice &L = e->lats[i];
+    if (L.Z > kZMAX) throw Error(HSMD_ERR_ARG, "too many neighbour slots");
+    e->nb[i].upload(L.g_nb.data(), L.g_nb.size());
+    e->bi[i].upload(L.g_bi.data(), L.g_bi.size());
+    e->bj[i].upload(L.g_bj.data(), L.g_bj.size());
+    e->bc[i].upload(L.g_bc.data(), L.g_bc.size());
+    LatDev &d = hl[i];
+    Coup &cp = d.cp;
+    cp.N = L.N; cp.Npad = L.N + 1; cp.Z = L.Z; cp.general = L.general; cp.h_on = L.h_on; cp.h = L.c.h;
+    for (int s = 0; s < kZMAX; s++) { cp.jxy[s] = cp.jz[s] = 0; cp.lam[s] = 1; }
+    for (int s = 0; s < L.Z; s++) {
+      const int sh = L.slot_shell[s];
+      cp.jxy[s] = L.shellJ[sh - 1];
+      cp.jz[s] = L.shellJ[sh - 1] * L.shellLam[sh - 1];
+      cp.lam[s] = L.shellLam[sh - 1];
+    }
+    d.bd.Nb = (int)L.g_bi.size();
+    d.bd.n_class = (int)L.bc_Jxy.size();
+    for (int k = 0; k < kBCMAX; k++) d.bd.jxy[k] = d.bd.jz[k] = 0;
+    for (int k = 0; k < d.bd.n_class; k++) { d.bd.jxy[k] = L.bc_Jxy[k]; d.bd.jz[k] = L.bc_Jz[k]; }
+    d.Jt = nullptr;
+    if (L.general) {
+      if (e->f32) {
+        std::vector<float> t(L.g_J.begin(), L.g_J.end());
+        e->Jf[i].upload(t.data(), t.size());
+        d.Jt = e->Jf[i].p;
+      } else {
+        e->Jd[i].upload(L.g_J.data(), L.g_J.size());
+        d.Jt = e->Jd[i].p;
+      }
+    }
+    d.nb = e->nb[i].p; d.bi = e->bi[i].p; d.bj = e->bj[i].p; d.bc = e->bc[i].p;
+    // DFT grid in new labels; vacant sites are masked out of the transform
+    if (!L.gridMap.empty()) {
+      std::vector<int32_t> g(L.gridMap.size());
+      for (size_t k = 0; k < g.size(); k++) {
+        const int site = L.gridMap[k];
+        g[k] = (site < 0 || L.vacant[site]) ? -1 : L.inv[site];
+      }
+      e->gmap[i].upload(g.data(), g.size());
+    } else if (sp.dssf) {
+      throw Error(HSMD_ERR_ARG, "DSSF on tabulated graphs needs the direct-sum transform (not built)");
+    }
+  }
+  e->dlats.upload(hl.data(), hl.size());
+  // sizes and layout
+  const int T_ = sp.n_steps;
+  e->Tc = std::min(T_, 256);
+  const int no = sp.n_omega;
+  long long so = 0, to = 0, yo = 0;
+  e->hs.resize(S);
+  for (int s = 0; s < S; s++) {
+    const int N = e->lats[sample_lattice[s]].N;
+    e->hs[s] = {sample_lattice[s], N, so, to, yo};
+    so += 3LL * (N + 1);
+    to += (long long)e->Tc * 3 * N;
+    yo += 3LL * N * 2 * no;
+  }
+  e->dsamples.upload(e->hs.data(), S);
+  e->spins.alloc(so * es);
+  e->spins.zero();
+  e->s0.alloc(so * es);
+  e->qss.alloc((size_t)S * T_);
+  e->qss.zero();
+  e->eref.alloc(S);
+  if (sp.dssf) {
+    e->traj.alloc(to * es);
+    e->ybuf.alloc(yo * es);
+    e->ybuf.zero();
+    if (e->f32) upload_twiddles<float>(*e); else upload_twiddles<double>(*e);
+  }
+  // launch shape
+  e->spt = 1;
+  while (e->spt * kSmdThreads < maxN) e->spt *= 2;
+  if (e->spt > 8) throw Error(HSMD_ERR_ARG, "lattice too large for the SMD kernel (N > 8192)");
+  e->lds_npad = maxN + 1;
+  e->lds_bytes = (size_t)2 * 3 * e->lds_npad * es;
+  e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024;
+  if (!e->use_lds) {
+    e->stage.alloc(so * es);
+    e->stage.zero();
+  }
+  return e.release();
+}
+
+void smd_set_samples(SMDEngine &e, const double *spins) {
+  std::vector<double> h64;
+  std::vector<float> h32;
+  const size_t total = e.spins.n / (e.f32 ? sizeof(float) : sizeof(double));
+  if (e.f32) h32.assign(total, 0.f); else h64.assign(total, 0.0);
+  size_t in = 0;
+  for (int s = 0; s < e.S; s++) {
+    const Lattice &L = e.lats[e.hs[s].li];
+    const int N = L.N, Npad = N + 1;
+    const size_t off = (size_t)e.hs[s].spin_off;
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++) {
+        const double v = spins[in + (size_t)i * 3 + d];
+        if (e.f32) h32[off + (size_t)d * Npad + L.inv[i]] = (float)v;
+        else h64[off + (size_t)d * Npad + L.inv[i]] = v;
+      }
+    in += (size_t)N * 3;
+  }
+  if (e.f32) HIP_CHECK(hipMemcpy(e.spins.p, h32.data(), total * 4, hipMemcpyHostToDevice));
+  else HIP_CHECK(hipMemcpy(e.spins.p, h64.data(), total * 8, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(e.s0.p, e.spins.p, e.spins.n, hipMemcpyDeviceToDevice));
+  e.t_next = 0;
+  e.chunk_base = 0;
+  e.chunk_fill = 0;
+  e.dssf_done = false;
+  e.qss.zero();
+  if (e.spec.dssf) e.ybuf.zero();
+  if (e.f32)
+    hipLaunchKernelGGL(smd_energy_kernel<float>, dim3(e.S), dim3(kSmdThreads), 0, 0, e.dlats.p,
+                       e.dsamples.p, (const float *)e.spins.p, e.eref.p);
+  else
+    hipLaunchKernelGGL(smd_energy_kernel<double>, dim3(e.S), dim3(kSmdThreads), 0, 0, e.dlats.p,
+                       e.dsamples.p, (const double *)e.spins.p, e.eref.p);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  e.set = true;
+}
+
+void smd_get_samples(SMDEngine &e, double *spins) {
+  const size_t total = e.spins.n / (e.f32 ? sizeof(float) : sizeof(double));
+  std::vector<double> h64;
+  std::vector<float> h32;
+  if (e.f32) { h32.resize(total); HIP_CHECK(hipMemcpy(h32.data(), e.spins.p, total * 4, hipMemcpyDeviceToHost)); }
+  else { h64.resize(total); HIP_CHECK(hipMemcpy(h64.data(), e.spins.p, total * 8, hipMemcpyDeviceToHost)); }
+  size_t o = 0;
+  for (int s = 0; s < e.S; s++) {
+    const Lattice &L = e.lats[e.hs[s].li];
+    const int N = L.N, Npad = N + 1;
+    const size_t off = (size_t)e.hs[s].spin_off;
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++)
+        spins[o + (size_t)i * 3 + d] = e.f32 ? (double)h32[off + (size_t)d * Npad + L.inv[i]]
+                                             : h64[off + (size_t)d * Npad + L.inv[i]];
+    o += (size_t)N * 3;
+  }
+}
+
+template <typename T, int SPT>
+static void launch_smd(SMDEngine &e, StepArgs a) {
+  if (e.use_lds) {
+    const void *fn = (const void *)smd_kernel<T, SPT, true>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+    a.sample_base = 0;
+    hipLaunchKernelGGL((smd_kernel<T, SPT, true>), dim3(e.S), dim3(kSmdThreads), e.lds_bytes, 0, a);
+  } else {
+    // one instance per launch, spins and stage buffer in global memory
+    for (int s = 0; s < e.S; s++) {
+      a.sample_base = s;
+      hipLaunchKernelGGL((smd_kernel<T, SPT, false>), dim3(1), dim3(kSmdThreads), 0, 0, a);
+    }
+  }
+}
+
+template <typename T>
+static void launch_smd_t(SMDEngine &e, const StepArgs &a) {
+  switch (e.spt) {
+    case 1: launch_smd<T, 1>(e, a); break;
+    case 2: launch_smd<T, 2>(e, a); break;
+    case 4: launch_smd<T, 4>(e, a); break;
+    default: launch_smd<T, 8>(e, a); break;
+  }
+}
+
+static void flush_chunk(SMDEngine &e) {
+  if (!e.spec.dssf || e.chunk_fill == 0) return;
+  const int no = e.spec.n_omega;
+  int maxM = 0;
+  for (auto &s : e.hs) maxM = std::max(maxM, 3 * s.N);
+  const int ldb = 2 * no;
+  dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
+  if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
+  if (e.f32)
+    hipLaunchKernelGGL(time_gemm_kernel<float>, grid, dim3(256), 0, 0, e.dsamples.p,
+                       (const float *)e.traj.p, (const float *)e.twid.p, ldb, e.chunk_base,
+                       e.chunk_fill, (float *)e.ybuf.p);
+  else
+    hipLaunchKernelGGL(time_gemm_kernel<double>, grid, dim3(256), 0, 0, e.dsamples.p,
+                       (const double *)e.traj.p, (const double *)e.twid.p, ldb, e.chunk_base,
+                       e.chunk_fill, (double *)e.ybuf.p);
+  HIP_CHECK(hipGetLastError());
+  e.chunk_base += e.chunk_fill;
+  e.chunk_fill = 0;
+}
+
+void smd_step(SMDEngine &e, int n) {
+  if (!e.set) throw Error(HSMD_ERR_STATE, "hsmd_smd_step before hsmd_smd_set_samples");
+  if (n < 0) throw Error(HSMD_ERR_ARG, "n_steps must be >= 0");
+  // the first call also records t = 0
+  int points = n + (e.t_next == 0 ? 1 : 0);
+  if (e.t_next + points > e.spec.n_steps)
+    throw Error(HSMD_ERR_ARG, "more steps than n_steps - 1");
+  while (points > 0) {
+    const int room = e.Tc - e.chunk_fill;
+    const int np = std::min(points, room);
+    StepArgs a{};
+    a.lats = e.dlats.p;
+    a.samples = e.dsamples.p;
+    a.spins = e.spins.p;
+    a.s0 = e.s0.p;
+    a.traj = e.spec.dssf ? (void *)e.traj.p : nullptr;
+    a.qss = e.qss.p;
+    a.eref = e.eref.p;
+    a.dt = e.spec.dt;
+    a.t0 = e.t_next;
+    a.n_points = np;
+    a.chunk_base = e.chunk_base;
+    a.T = e.spec.n_steps;
+    a.lds_npad = e.lds_npad;
+    a.stage = e.stage.p;
+    if (e.f32) launch_smd_t<float>(e, a); else launch_smd_t<double>(e, a);
+    HIP_CHECK(hipGetLastError());
+    e.t_next += np;
+    e.chunk_fill += np;
+    points -= np;
+    if (e.chunk_fill == e.Tc) flush_chunk(e);
+  }
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+template <typename T>
+static void dssf_space(SMDEngine &e, int li, double *out) {
+  const Lattice &L = e.lats[li];
+  const int G = L.Gsz, no = e.spec.n_omega, Nq = G * G;
+  std::vector<int> ids;
+  for (int s = 0; s < e.S; s++) if (e.sample_lat[s] == li) ids.push_back(s);
+  DevBuf<double> d;
+  d.alloc((size_t)3 * Nq * no);
+  d.zero();
+  std::vector<T> cs(G), sn(G);
+  const double tp = 2.0 * acos(-1.0);
+  for (int k = 0; k < G; k++) { cs[k] = (T)cos(tp * k / G); sn[k] = (T)sin(tp * k / G); }
+  DevBuf<T> dcs, dsn;
+  dcs.upload(cs.data(), G);
+  dsn.upload(sn.data(), G);
+  const int batch = std::max(1, std::min((int)ids.size(), 16));
+  DevBuf<T> R;
+  R.alloc((size_t)batch * 3 * Nq * 2 * no);
+  DevBuf<int> dids;
+  const double scale = 1.0 / ((double)Nq * (double)e.spec.n_steps);
+  for (size_t b0 = 0; b0 < ids.size(); b0 += batch) {
+    const int nb = std::min(batch, (int)(ids.size() - b0));
+    dids.upload(ids.data() + b0, nb);
+    hipLaunchKernelGGL(dft_rows_kernel<T>, dim3(G, 3, nb), dim3(256), 0, 0, e.dsamples.p, dids.p,
+                       e.gmap[li].p, G, no, (const T *)e.ybuf.p, dcs.p, dsn.p, R.p);
+    hipLaunchKernelGGL(dft_cols_kernel<T>, dim3(G, 3), dim3(256), 0, 0, nb, G, no, R.p, dcs.p,
+                       dsn.p, scale, d.p);
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipMemcpy(out, d.p, (size_t)3 * Nq * no * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void smd_dssf(SMDEngine &e, int li, double *out, double *qss) {
+  if (li < 0 || li >= (int)e.lats.size()) throw Error(HSMD_ERR_ARG, "bad lattice index");
+  if (!e.spec.dssf) throw Error(HSMD_ERR_STATE, "DSSF accumulation is off");
+  if (e.t_next != e.spec.n_steps) throw Error(HSMD_ERR_STATE, "DSSF needs all n_steps time points");
+  flush_chunk(e);
+  if (e.f32) dssf_space<float>(e, li, out); else dssf_space<double>(e, li, out);
+  if (qss) {
+    const int T_ = e.spec.n_steps;
+    std::vector<double> q((size_t)e.S * T_);
+    HIP_CHECK(hipMemcpy(q.data(), e.qss.p, q.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 0; t < T_; t++) qss[t] = 0.0;
+    for (int s = 0; s < e.S; s++)
+      if (e.sample_lat[s] == li)
+        for (int t = 0; t < T_; t++) qss[t] += q[(size_t)s * T_ + t];
+  }
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+void smd_energies(SMDEngine &e, double *E) {
+  HIP_CHECK(hipMemcpy(E, e.eref.p, e.S * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void smd_destroy(SMDEngine *e) { delete e; }
+
+}  // namespace hsmd

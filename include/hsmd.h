@@ -1,0 +1,219 @@
+/*
+ * hsmd.h -- C ABI of the MI355X-native Monte Carlo sweep engine and
+ * semiclassical spin-dynamics (SMD) engine for 2D Heisenberg / Ising models.
+ *
+ * The reference (rmpt-phys/Classical_MC_and_SMD_for_2D_Heisenberg_Models) has
+ * no library boundary: its hot path is a set of functions over mutable globals
+ * called from two monolithic `main`s.  Each entry point below names the
+ * reference function or inline block it replaces (paths relative to the
+ * reference root, CF = Codes_Folder).
+ *
+ * ABI rules
+ *   - every function returns an hsmd_status (0 = success); on failure the
+ *     message is kept per thread and read back with hsmd_last_error();
+ *   - all buffers are caller-owned host memory; the library copies in and out
+ *     and keeps device memory opaque;
+ *   - spins cross the ABI as FP64 (x,y,z) triples in site order, the layout of
+ *     the reference's binary sample files (CF/mc_hm_code.cpp:5167-5177);
+ *   - no C++ exception crosses the ABI; handles are not thread-safe (one
+ *     handle per host thread), but distinct handles are independent.
+ */
+#ifndef HSMD_H
+#define HSMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int hsmd_status;
+enum {
+  HSMD_OK = 0,
+  HSMD_ERR_ARG = 1,     /* invalid argument / shape */
+  HSMD_ERR_GUARD = 2,   /* a reference guard rail rejected the input */
+  HSMD_ERR_DEVICE = 3,  /* HIP runtime error */
+  HSMD_ERR_IO = 4,      /* file missing / wrong size */
+  HSMD_ERR_STATE = 5    /* call out of order (e.g. run before set_spins) */
+};
+
+/* Precision / batching forms of the MC and SMD engines. */
+enum {
+  HSMD_PLAIN_F64 = 0,    /* FP64, one instance per launch, global memory */
+  HSMD_BATCHED_F32 = 1,  /* FP32 state, FP64 reductions, all instances in one launch */
+  HSMD_BATCHED_F64 = 2   /* FP64 state, all instances in one launch */
+};
+
+enum { HSMD_HEISENBERG = 0, HSMD_ISING = 1 };
+
+/* Coupling constants, one-to-one with Settings_MODEL.txt / Settings_Disorder.txt
+ * (Set_Params.py:6-72; parsed at CF/subrouts.cpp:955-2186). */
+typedef struct hsmd_couplings {
+  double J1, J2, JX;      /* crystal-crystal couplings per shell */
+  double J2_delta;        /* Lieb-lattice J2 asymmetry (requires geometry "lieb") */
+  double lambda1, lambda2;/* XXZ factors: shell 1 uses lambda1, shells 2 and X use lambda2 */
+  double h;               /* external field along z (dMag = 2) */
+  double imp_ratio;       /* impurity ratio in [0,1); 0 disables disorder */
+  double J1ic, J2ic, JXic;/* impurity-crystal couplings */
+  double J1ii, J2ii, JXii;/* impurity-impurity couplings */
+} hsmd_couplings;
+
+typedef struct hsmd_lattice hsmd_lattice;
+typedef struct hsmd_mc hsmd_mc;
+typedef struct hsmd_smd hsmd_smd;
+
+/* Copies the last error message of the calling thread (NUL-terminated). */
+hsmd_status hsmd_last_error(char *buf, size_t len);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int hsmd_device_count(void);
+
+/* ------------------------------------------------------------------ lattice */
+
+/* Builds a periodic lattice with the reference's site numbering, neighbour
+ * shells and in-shell slot order (replaces make_rvecList CF/subrouts.cpp:2714,
+ * make_nborsTable CF/subrouts.cpp:2983, make_bondList CF/subrouts.cpp:3426).
+ * geometry: "square", "triang", "hexagn", "kagome", "lieb".
+ * pbc = 0 turns image neighbours into -1 holes (the reference's PBC_OFF).
+ * strict = 1 applies the reference's guard rails (L even; L % 3 == 0 for
+ * C3-symmetric geometries); strict = 0 is the extended mode. */
+hsmd_status hsmd_lattice_generate(const char *geometry, int L, int pbc, int strict,
+                                  const hsmd_couplings *c, hsmd_lattice **out);
+
+/* Builds a lattice from a tabulated neighbour table: per site zn1 shell-1,
+ * then zn2 shell-2, then znx shell-X int32 entries, -1 = missing (the
+ * nborsList*.bin layout written at CF/subrouts.cpp:3294-3316, read at
+ * CF/mc_hm_code.cpp:1273-1320).  rvec (n x 2 f64, may be NULL) is the
+ * sitesList*.bin layout (CF/mc_hm_code.cpp:955-1020).  grid_L > 0 sets the
+ * Lsz x Lsz direct-sum q-grid of get_qct_SqData (CF/subrouts.cpp:8460). */
+hsmd_status hsmd_lattice_from_tables(int32_t n, int zn1, int zn2, int znx,
+                                     const int32_t *nbors, const double *rvec,
+                                     const hsmd_couplings *c, hsmd_lattice **out);
+
+/* Sets the impurity map (n int32 of 0/1; CF/subrouts.cpp:3321-3352 draws it). */
+hsmd_status hsmd_lattice_set_impurities(hsmd_lattice *lat, const int32_t *imp);
+
+/* Removes sites: every table entry pointing at a vacant site becomes -1 and
+ * the vacant site loses all its own entries (a vacancy is a site whose
+ * couplings are all zero). */
+hsmd_status hsmd_lattice_set_vacancies(hsmd_lattice *lat, const int32_t *vacant);
+
+/* info[16]: n, zn1, zn2, znx, n_bonds, n_colours, Gsz, Nq, nsubs, L,
+ *           j2_on, jx_on, xxz_on, h_on, disorder, c3sym */
+hsmd_status hsmd_lattice_info(const hsmd_lattice *lat, int32_t info[16]);
+hsmd_status hsmd_lattice_tables(const hsmd_lattice *lat, int32_t *nbors, double *rvec);
+hsmd_status hsmd_lattice_grid_map(const hsmd_lattice *lat, int32_t *grid_map);
+/* bonds: n_bonds x 2 int32 pairs (i, j) in make_bondList order; shell: n_bonds int32 (1,2,3). */
+hsmd_status hsmd_lattice_bonds(const hsmd_lattice *lat, int32_t *pairs, int32_t *shell);
+/* colour of each site (0..n_colours-1) used by the GPU sweeps. */
+hsmd_status hsmd_lattice_colours(const hsmd_lattice *lat, int32_t *colour);
+void hsmd_lattice_destroy(hsmd_lattice *lat);
+
+/* --------------------------------------------------- fields and energies */
+
+/* Local fields of R spin configurations (R x n x 3 f64 in, R x n x 3 f64 out);
+ * replaces get_localField (CF/subrouts.cpp:4465) evaluated at every site.
+ * precision: HSMD_PLAIN_F64 / HSMD_BATCHED_F64 (FP64) or HSMD_BATCHED_F32. */
+hsmd_status hsmd_local_fields(const hsmd_lattice *lat, int precision, int R,
+                              const double *spins, double *fields);
+
+/* Total energies of R configurations (R f64); replaces get_energyValue
+ * (CF/subrouts.cpp:5181).  The reduction is FP64 in every precision form. */
+hsmd_status hsmd_energies(const hsmd_lattice *lat, int precision, int R,
+                          const double *spins, double *energies);
+
+/* ---------------------------------------------------------------- MC engine */
+
+typedef struct hsmd_mc_spec {
+  int model;        /* HSMD_HEISENBERG or HSMD_ISING */
+  int precision;    /* HSMD_PLAIN_F64 / HSMD_BATCHED_F32 / HSMD_BATCHED_F64 */
+  int n_or, n_hb;   /* overrelaxation and heat-bath passes per sweep (reference: 5, 5) */
+  int n_or_boost, n_hb_boost; /* passes in the sweep after an accepted swap (reference: 2, 8) */
+  int pt_cap50;     /* 1: reference 50 % acceptance-ratio cap (CF/mc_hm_code.cpp:3099) */
+  uint64_t seed;
+} hsmd_mc_spec;
+
+/* R replicas, temps[R] per temperature slot (the reference's rank order). */
+hsmd_status hsmd_mc_create(const hsmd_lattice *lat, int R, const double *temps,
+                           const hsmd_mc_spec *spec, hsmd_mc **out);
+/* Spins of the configuration at each temperature slot (R x n x 3 f64).
+ * Ising spins are (+-1, 0, 0) as in the reference (CF/subrouts.cpp:4695-4725). */
+hsmd_status hsmd_mc_set_spins(hsmd_mc *mc, const double *spins);
+hsmd_status hsmd_mc_get_spins(hsmd_mc *mc, double *spins);
+
+/* Runs n_sweeps sweeps of every replica; replaces the sweep loops
+ * CF/mc_hm_code.cpp:2778-2962 (thermalisation) and 4836-4945 (measurement)
+ * and the MPI replica exchange CF/mc_hm_code.cpp:3015-3130 / 5253-5365.
+ * pt_parity: -1 = no exchange; 2 = alternate pair parity with the engine's
+ * sweep counter (thermalisation); 0 or 1 = fixed parity (measurement uses
+ * the bin index).  measure = 1 accumulates the per-slot observable moments.
+ * energies (nullable): n_sweeps x R f64, the energy at each slot after each
+ * sweep, before the exchange. */
+hsmd_status hsmd_mc_run(hsmd_mc *mc, int n_sweeps, int pt_parity, int measure,
+                        double *energies);
+/* Energies per temperature slot of the current configurations. */
+hsmd_status hsmd_mc_energies(hsmd_mc *mc, double *energies);
+/* attempts[R], accepts[R]: the reference's per-rank ptCnt / ptExc counters. */
+hsmd_status hsmd_mc_pt_counters(hsmd_mc *mc, int64_t *attempts, int64_t *accepts);
+/* Resets the exchange counters and the sweep counter (CF/mc_hm_code.cpp:4767). */
+hsmd_status hsmd_mc_reset_counters(hsmd_mc *mc);
+/* Observable moments accumulated per slot since the last reset:
+ * out[R x HSMD_N_MOMENTS], n_meas = number of measured sweeps.
+ * Moment order: E, E^2, F1, F2, F4, T1, T2, Q1, Q2, Q4, S1, S2, S4, SH, SV, SD,
+ *               X1, X2, X4 (X = C3 Psi or Z2 Ising parameter). */
+#define HSMD_N_MOMENTS 19
+hsmd_status hsmd_mc_moments(hsmd_mc *mc, double *out, int64_t *n_meas, int reset);
+/* Slot -> configuration map (R int32). */
+hsmd_status hsmd_mc_slot_map(hsmd_mc *mc, int32_t *config_of_slot);
+void hsmd_mc_destroy(hsmd_mc *mc);
+
+/* -------------------------------------------------------------- SMD engine */
+
+typedef struct hsmd_smd_spec {
+  int precision;    /* HSMD_PLAIN_F64 / HSMD_BATCHED_F32 / HSMD_BATCHED_F64 */
+  double dt;        /* time step dtm */
+  int n_steps;      /* ntm: number of time points, including t = 0 */
+  int n_omega;      /* frequency bins kept, n < n_omega (n_omega = n_steps keeps all) */
+  int dssf;         /* 1: accumulate the DSSF; 0: dynamics only */
+  int gauss_seidel; /* 1: in-place sequential energy correction (reference order,
+                       CPU only); 0: Jacobi (GPU) */
+} hsmd_smd_spec;
+
+/* S samples; sample s evolves on lattices[sample_lattice[s]]. window[n_steps]
+ * is the temporal window (nullable = "nowin"). */
+hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
+                            const int32_t *sample_lattice, int S,
+                            const hsmd_smd_spec *spec, const double *window,
+                            hsmd_smd **out);
+/* Spins of every sample, concatenated in sample order (sum_s n_s x 3 f64). */
+hsmd_status hsmd_smd_set_samples(hsmd_smd *smd, const double *spins);
+hsmd_status hsmd_smd_get_samples(hsmd_smd *smd, double *spins);
+/* Advances every sample n_steps RK4 steps (each with its energy correction);
+ * replaces the time loop of get_dynSpectrum (CF/subrouts.cpp:11734-11927).
+ * The first call records t = 0.  The sum of calls must not exceed
+ * spec.n_steps - 1 when the DSSF is on. */
+hsmd_status hsmd_smd_step(hsmd_smd *smd, int n_steps);
+/* Sample-summed DSSF of lattice li: out[3][Nq][n_omega] f64 =
+ * sum_s |X_s(q, w)|^2 with X the windowed, 1/sqrt(Nq T)-normalised space-time
+ * DFT (CF/subrouts.cpp:11906-11925, 12062-12105), and qss[n_steps] the
+ * sample-summed autocorrelation (1/N) sum_i S_i(0).S_i(t).  Available after
+ * n_steps - 1 steps. Divide by the sample count for the reference's averages. */
+hsmd_status hsmd_smd_dssf(hsmd_smd *smd, int li, double *out, double *qss);
+/* Energies of every sample (S f64). */
+hsmd_status hsmd_smd_energies(hsmd_smd *smd, double *energies);
+void hsmd_smd_destroy(hsmd_smd *smd);
+
+/* --------------------------------------------------------- self checks */
+
+/* Writes n Philox4x32-10 blocks for counters (site = i, step, inst, stream)
+ * computed on the device (4n uint32), for the bit-exact stream check. */
+hsmd_status hsmd_debug_philox(uint64_t seed, uint32_t step, uint32_t inst,
+                              uint32_t stream, int n, uint32_t *out);
+/* Device det_exp_neg on n inputs, for the bit-exact acceptance check. */
+hsmd_status hsmd_debug_exp(int n, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSMD_H */

@@ -1,0 +1,220 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/cpu_ref.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpu_ref.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_I64 = ctypes.POINTER(ctypes.c_longlong)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+_SIG = {
+    "orc_lattice_dims": (_I, [ctypes.c_char_p, _I] + [ctypes.POINTER(_I)] * 5),
+    "orc_lattice_generate": (_I, [ctypes.c_char_p, _I, _I, _I32, _D, _I32]),
+    "orc_model_create": (_P, [_I, _I, _I, _I, _I32, _I32, _D, _I]),
+    "orc_model_destroy": (None, [_P]),
+    "orc_bond_count": (_I, [_P]),
+    "orc_bonds": (None, [_P, _I32, _I32]),
+    "orc_local_fields": (None, [_P, _I, _D, _D]),
+    "orc_local_fields_x": (None, [_P, _D, _D]),
+    "orc_energies": (None, [_P, _I, _D, _D]),
+    "orc_energies_ising": (None, [_P, _I, _D, _D]),
+    "orc_reflect": (None, [_D, _D, _D]),
+    "orc_heat_bath": (None, [ctypes.c_double, _D, ctypes.c_double, ctypes.c_double, _D]),
+    "orc_philox": (None, [ctypes.c_uint64, _U32, _U32]),
+    "orc_det_exp": (ctypes.c_double, [ctypes.c_double]),
+    "orc_mc_replay": (None, [_P, _I, _I, _D, _I32, _I, _I, _I, _I, _I, ctypes.c_uint64,
+                             ctypes.c_uint32, _I, _I, _D, _I32, _I64, _I64, _I32, _D]),
+    "orc_mc_reference": (None, [_P, _I, _I, _D, _I, _I, _I, _I, _I, ctypes.c_uint64, _I, _I,
+                                _D, _I32, _D, _I]),
+    "orc_smd_run": (None, [_P, _D, ctypes.c_double, _I, _I, _D, _D, _D]),
+    "orc_dssf": (None, [_I, _I, _I32, _I, _D, _D, _I, _D]),
+    "orc_window": (_I, [ctypes.c_char_p, _I, _D]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for k, (r, a) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I32)
+
+
+def lattice(geom, L, pbc=True):
+    """Reference lattice by the reference's full scan: (nbors, rvec, gridmap, zn)."""
+    N, z1, z2, zx, G = (_I(), _I(), _I(), _I(), _I())
+    rc = lib().orc_lattice_dims(geom.encode(), L, ctypes.byref(N), ctypes.byref(z1),
+                                ctypes.byref(z2), ctypes.byref(zx), ctypes.byref(G))
+    if rc:
+        raise ValueError("unknown geometry")
+    zn = (z1.value, z2.value, zx.value)
+    nb = np.zeros((N.value, sum(zn)), np.int32)
+    rv = np.zeros((N.value, 2))
+    gm = np.zeros(G.value * G.value, np.int32)
+    rc = lib().orc_lattice_generate(geom.encode(), L, int(pbc), _ip(nb), _dp(rv), _ip(gm))
+    if rc:
+        raise ValueError("oracle lattice generation failed (%d)" % rc)
+    return nb, rv, gm, zn
+
+
+class Model:
+    """Couplings + neighbour table (params as Couplings.as_params order)."""
+
+    def __init__(self, nbors, zn, params, imp=None, lieb=False):
+        self.nb = np.ascontiguousarray(nbors, dtype=np.int32)
+        self.N = self.nb.shape[0]
+        self.zn = zn
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        self.imp = None if imp is None else np.ascontiguousarray(imp, dtype=np.int32)
+        self._h = lib().orc_model_create(self.N, zn[0], zn[1], zn[2], _ip(self.nb),
+                                         None if self.imp is None else _ip(self.imp),
+                                         _dp(self.params), int(lieb))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_model_destroy(self._h)
+            self._h = None
+
+    def bonds(self):
+        n = lib().orc_bond_count(self._h)
+        p = np.zeros((n, 2), np.int32)
+        s = np.zeros(n, np.int32)
+        lib().orc_bonds(self._h, _ip(p), _ip(s))
+        return p, s
+
+    def local_fields(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, self.N, 3)
+        out = np.zeros_like(s)
+        lib().orc_local_fields(self._h, s.shape[0], _dp(s), _dp(out))
+        return out
+
+    def local_fields_x(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.N, 3)
+        out = np.zeros(self.N)
+        lib().orc_local_fields_x(self._h, _dp(s), _dp(out))
+        return out
+
+    def energies(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, self.N, 3)
+        out = np.zeros(s.shape[0])
+        lib().orc_energies(self._h, s.shape[0], _dp(s), _dp(out))
+        return out
+
+    def energies_ising(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, self.N, 3)
+        out = np.zeros(s.shape[0])
+        lib().orc_energies_ising(self._h, s.shape[0], _dp(s), _dp(out))
+        return out
+
+    def mc_replay(self, model, temps, colour, spins, n_sweeps, n_or=5, n_hb=5, n_or_b=2, n_hb_b=8,
+                  cap50=True, seed=1, sweep0=0, pt_parity=-1, state=None):
+        """Colour-order replay of the GPU sweep; spins (R, N, 3) are updated in
+        place (configuration order).  state = (cos, cnt, exc, boost)."""
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        R = len(temps)
+        if state is None:
+            state = (np.arange(R, dtype=np.int32), np.zeros(R, np.int64), np.zeros(R, np.int64),
+                     np.zeros(R, np.int32))
+        cos, cnt, exc, boost = state
+        tr = np.zeros((n_sweeps, R))
+        col = np.ascontiguousarray(colour, dtype=np.int32)
+        lib().orc_mc_replay(self._h, model, R, _dp(temps), _ip(col), n_or, n_hb, n_or_b, n_hb_b,
+                            int(cap50), seed, sweep0, n_sweeps, pt_parity, _dp(spins), _ip(cos),
+                            cnt.ctypes.data_as(_I64), exc.ctypes.data_as(_I64), _ip(boost),
+                            _dp(tr))
+        return tr, state
+
+    def mc_reference(self, model, temps, spins, n_sweeps, n_or=5, n_hb=5, n_or_b=2, n_hb_b=8,
+                     cap50=True, seed=1, pt_start=100, threads=0):
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        R = len(temps)
+        cos = np.arange(R, dtype=np.int32)
+        acc = np.zeros((R, 2))
+        lib().orc_mc_reference(self._h, model, R, _dp(temps), n_or, n_hb, n_or_b, n_hb_b,
+                               int(cap50), seed, n_sweeps, pt_start, _dp(spins), _ip(cos),
+                               _dp(acc), threads)
+        return cos, acc
+
+    def smd_run(self, spins, dt, n_points, gauss_seidel=True, traj=True):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.N, 3).copy()
+        tr = np.zeros((n_points, self.N, 3)) if traj else None
+        q = np.zeros(n_points)
+        e = np.zeros(n_points)
+        lib().orc_smd_run(self._h, _dp(s), dt, n_points, int(gauss_seidel),
+                          None if tr is None else _dp(tr), _dp(q), _dp(e))
+        return s, tr, q, e
+
+
+def dssf(traj, gridmap, G, window, n_omega):
+    traj = np.ascontiguousarray(traj, dtype=np.float64)
+    T, N = traj.shape[0], traj.shape[1]
+    gm = np.ascontiguousarray(gridmap, dtype=np.int32)
+    w = np.ascontiguousarray(window, dtype=np.float64)
+    out = np.zeros((3, G * G, n_omega))
+    lib().orc_dssf(N, G, _ip(gm), T, _dp(traj), _dp(w), n_omega, _dp(out))
+    return out
+
+
+def window(name, n):
+    w = np.zeros(n)
+    if lib().orc_window(name.encode(), n, _dp(w)):
+        raise ValueError("unknown window " + name)
+    return w
+
+
+def philox(seed, ctr):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().orc_philox(seed, c.ctypes.data_as(_U32), o.ctypes.data_as(_U32))
+    return o
+
+
+def det_exp(x):
+    return lib().orc_det_exp(float(x))
+
+
+def reflect(s, h):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    o = np.zeros(3)
+    lib().orc_reflect(_dp(s), _dp(h), _dp(o))
+    return o
+
+
+def heat_bath(beta, h, u1, u2):
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    o = np.zeros(3)
+    lib().orc_heat_bath(beta, _dp(h), u1, u2, _dp(o))
+    return o

@@ -1,0 +1,201 @@
+"""GPU parity tests of the MC engine through the C ABI, against the oracle."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import classical_mc_and_smd_for_2d_heisenberg_models_amd as H
+import cpu_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-mc]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _spins(rng, R, N):
+    v = rng.normal(size=(R, N, 3))
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _model(lat, c, geom=None, imp=None):
+    return cpu_ref.Model(lat.tables(), (lat.zn1, lat.zn2, lat.znx), c.as_params(), imp=imp,
+                         lieb=(geom == "lieb"))
+
+
+def test_philox_and_exp_are_bit_identical_to_oracle():
+    _say("philox/exp")
+    seed = 0x1234_5678_9abc_def0
+    g = H.debug_philox(seed, 77, 5, 1, 1000)
+    for i in [0, 1, 17, 999]:
+        np.testing.assert_array_equal(g[i], cpu_ref.philox(seed, [i, 77, 5, 1]))
+    rng = np.random.default_rng(0)
+    x = np.concatenate([-rng.random(5000) * 720, -rng.random(500) * 1e-3, [0.0, 2.0, -699.9]])
+    y = H.debug_exp(x)
+    ref = np.array([cpu_ref.det_exp(v) for v in x])
+    np.testing.assert_array_equal(y.view(np.uint64), ref.view(np.uint64))
+
+
+FIELD_CASES = [
+    ("square", 24, dict(J1=1.0, J2=0.6), None),
+    ("triang", 12, dict(J1=1.0, J2=0.12, JX=0.05, h=0.3, lambda1=0.9, lambda2=0.9), None),
+    ("kagome", 6, dict(J1=1.0, J2=-0.2, JX=0.1, lambda1=1.2, lambda2=0.8, h=-0.4), None),
+    ("hexagn", 6, dict(J1=0.7, J2=0.3, JX=0.2), None),
+    ("lieb", 6, dict(J1=1.0, J2=0.2, J2_delta=0.05), None),
+    ("square", 12, dict(J1=1.0, J2=0.5, imp_ratio=0.2, J1ic=0.5, J1ii=0.3, J2ic=0.1, J2ii=0.05), 0.2),
+]
+
+
+@pytest.mark.parametrize("geom,L,kw,imp", FIELD_CASES)
+def test_local_fields_and_energies_fp64_match_oracle(geom, L, kw, imp):
+    _say("fields", geom, L, kw)
+    c = H.Couplings(**kw)
+    lat = H.Lattice.generate(geom, L, c)
+    rng = np.random.default_rng(L)
+    impmap = None
+    if imp:
+        impmap = (rng.random(lat.N) < imp).astype(np.int32)
+        lat.set_impurities(impmap)
+    m = _model(lat, c, geom, impmap)
+    s = _spins(rng, 3, lat.N)
+    for prec in (H.BATCHED_F64, H.PLAIN_F64):
+        h = lat.local_fields(s, prec)
+        np.testing.assert_allclose(h, m.local_fields(s), rtol=1e-12, atol=1e-13)
+        E = lat.energies(s, prec)
+        np.testing.assert_allclose(E, m.energies(s), rtol=1e-12, atol=1e-11)
+    # FP32 state: fields to FP32 rounding, energies reduced in FP64
+    h32 = lat.local_fields(s, H.BATCHED_F32)
+    np.testing.assert_allclose(h32, m.local_fields(s), atol=2e-5)
+    s32 = s.astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(lat.energies(s, H.BATCHED_F32), m.energies(s32), rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("prec", [H.BATCHED_F32, H.PLAIN_F64])
+@pytest.mark.parametrize("case", ["triang_h", "square_disorder"])
+def test_ising_metropolis_is_bit_exact_against_colour_replay(prec, case):
+    _say("ising replay", prec, case)
+    rng = np.random.default_rng(21)
+    imp = None
+    if case == "triang_h":
+        geom, L, c = "triang", 12, H.Couplings(J1=1.0, J2=0.12, JX=0.05, h=0.3)
+        lat = H.Lattice.generate(geom, L, c)
+    else:
+        geom, L, c = "square", 10, H.Couplings(J1=1.0, J2=0.4, imp_ratio=0.3, J1ic=0.6, J1ii=-0.2,
+                                                J2ic=0.1, J2ii=0.3)
+        lat = H.Lattice.generate(geom, L, c)
+        imp = (rng.random(lat.N) < 0.3).astype(np.int32)
+        lat.set_impurities(imp)
+    m = _model(lat, c, geom, imp)
+    temps = np.geomspace(0.5, 5.0, 8)
+    R = len(temps)
+    s = np.zeros((R, lat.N, 3))
+    s[:, :, 0] = rng.choice([-1.0, 1.0], size=(R, lat.N))
+    mc = H.MonteCarlo(lat, temps, model=H.ISING, precision=prec, seed=99)
+    mc.set_spins(s)
+    np.testing.assert_array_equal(mc.energies(), m.energies_ising(s))
+    tr = mc.run(30, pt_parity=2, trace=True)
+    ref = s.copy()
+    rtr, (cos, cnt, exc, boost) = m.mc_replay(1, temps, lat.colours(), ref, 30, seed=99,
+                                              pt_parity=2)
+    np.testing.assert_array_equal(tr, rtr)
+    np.testing.assert_array_equal(mc.slot_map(), cos)
+    att, acc = mc.pt_counters()
+    np.testing.assert_array_equal(att, cnt)
+    np.testing.assert_array_equal(acc, exc)
+    np.testing.assert_array_equal(mc.get_spins(), ref[cos])
+    assert acc.sum() > 0
+
+
+@pytest.mark.parametrize("prec", [H.BATCHED_F64, H.PLAIN_F64])
+def test_heisenberg_fp64_sweep_matches_colour_replay(prec):
+    _say("heis fp64 replay", prec)
+    c = H.Couplings(J1=1.0, J2=0.6, h=0.2, lambda1=0.9, lambda2=1.1)
+    lat = H.Lattice.generate("square", 8, c)
+    m = _model(lat, c)
+    temps = np.array([0.3, 0.7, 1.5])
+    rng = np.random.default_rng(5)
+    s = _spins(rng, 3, lat.N)
+    mc = H.MonteCarlo(lat, temps, precision=prec, n_or=2, n_hb=2, seed=7)
+    mc.set_spins(s)
+    tr = mc.run(2, pt_parity=-1, trace=True)
+    ref = s.copy()
+    rtr, _ = m.mc_replay(0, temps, lat.colours(), ref, 2, n_or=2, n_hb=2, seed=7)
+    np.testing.assert_allclose(mc.get_spins(), ref, atol=1e-9)
+    np.testing.assert_allclose(tr, rtr, rtol=1e-10)
+
+
+def test_heisenberg_fp32_open_chain_energy_matches_exact():
+    # classical open chain: <S_i.S_i+1> = 1/(bJ) - coth(bJ) exactly; pins the
+    # FP32 heat bath + overrelaxation statistically
+    _say("open chain fp32")
+    N = 512
+    nb = -np.ones((N, 3), np.int32)
+    for i in range(N):
+        if i > 0: nb[i, 0] = i - 1
+        if i < N - 1: nb[i, 1] = i + 1
+    lat = H.Lattice.from_tables(nb, 2, 0, 1, None, H.Couplings(J1=1.0))
+    temps = np.array([0.25, 0.5, 1.0, 2.0])
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=1, n_hb=1, seed=3)
+    rng = np.random.default_rng(1)
+    mc.set_spins(_spins(rng, len(temps), N))
+    mc.run(500)
+    tr = mc.run(4000, trace=True)
+    for k, T in enumerate(temps):
+        b = 1.0 / T
+        exact = (N - 1) * (1 / b - 1 / math.tanh(b))
+        est = tr[:, k].mean()
+        # blocked error estimate
+        blocks = tr[:, k].reshape(40, -1).mean(axis=1)
+        err = blocks.std(ddof=1) / math.sqrt(len(blocks))
+        assert abs(est - exact) < 5 * err + 1e-3 * abs(exact), (T, est, exact, err)
+
+
+def test_heisenberg_fp32_matches_reference_order_statistics():
+    # square J1-J2 at three temperatures: GPU FP32 colour sweeps vs the
+    # oracle's reference-order FP64 chain, mean energies within error bars
+    _say("square fp32 vs reference order")
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 8, c)
+    m = _model(lat, c)
+    temps = np.array([0.3, 0.6, 1.2])
+    rng = np.random.default_rng(2)
+    s = _spins(rng, 3, lat.N)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, seed=11)
+    mc.set_spins(s)
+    mc.run(1000)
+    tr = mc.run(8000, trace=True)
+    ref = s.copy()
+    nref = 8000
+    m.mc_reference(0, temps, ref, 1000, n_or=5, n_hb=1, pt_start=10 ** 9, seed=5)
+    cos, acc = m.mc_reference(0, temps, ref, nref, n_or=5, n_hb=1, pt_start=10 ** 9, seed=6)
+    for k in range(3):
+        g = tr[:, k]
+        gb = g.reshape(40, -1).mean(axis=1)
+        ge = gb.std(ddof=1) / math.sqrt(40)
+        rmean = acc[k, 0] / nref
+        rvar = acc[k, 1] / nref - rmean ** 2
+        re = math.sqrt(rvar / nref) * 4  # autocorrelation allowance
+        assert abs(g.mean() - rmean) < 3 * math.hypot(ge, re) + 0.05, (temps[k], g.mean(), rmean)
+
+
+def test_pt_moves_configurations_and_respects_cap():
+    _say("pt counters")
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 8, c)
+    temps = np.geomspace(0.05, 1.5, 16)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, seed=1, pt_cap50=True)
+    rng = np.random.default_rng(0)
+    mc.set_spins(_spins(rng, 16, lat.N))
+    mc.run(300, pt_parity=2)
+    att, acc = mc.pt_counters()
+    assert att.min() > 0
+    # the receiver's ratio (acc+1)/att stays within the 50 % cap (+1 slack)
+    assert np.all(acc <= 0.5 * att + 2)
+    perm = mc.slot_map()
+    assert sorted(perm.tolist()) == list(range(16))
+    # energies stay ordered on average with temperature
+    E = mc.run(200, pt_parity=2, trace=True).mean(axis=0)
+    assert np.all(np.diff(E) > -2.0)

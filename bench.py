@@ -1,0 +1,293 @@
+"""Benchmark of the MC sweep engine and the SMD engine on one MI355X.
+
+Headline workload (value): batched parallel tempering, precision form --
+triangular L=96 (N=9216), J1=1, J2=0.12, JX=0.05, h=0.3, XXZ lambda=0.9,
+256 replicas geometric in T in [0.02, 2.0], FP32 spins and fields, FP64
+energies and exchange acceptance, one sweep = 5 overrelaxation + 1 heat-bath
+passes of every replica followed by the on-device replica exchange.  A step
+is one such sweep of all replicas; value = spin-updates/s.
+
+Also reported on the same line:
+  plain_f64      -- the same workload in the plain FP64 one-instance-per-launch
+                    form (a few sweeps), and the batched/plain ratio;
+  smd            -- batched uneven SMD: 512 samples over square L=64, kagome
+                    L=32, hexagonal L=40 and square L=48 with 10 % vacancies,
+                    FP32 RK4 (dt=0.005) with FP64 energy correction and the
+                    DSSF time contraction (512 frequency bins) included, in
+                    site-RK4-steps/s; and the plain FP64 SMD form on the
+                    L=24 square lattice with 32 samples;
+  roofline       -- the sweep kernel's algorithmic bytes per launch over its
+                    average launch duration (HIP events on the launch stream);
+  cpu_baseline   -- the oracle's reference-order FP64 sweeps (one thread per
+                    replica, like one MPI rank each) on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import classical_mc_and_smd_for_2d_heisenberg_models_amd as H  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+class HipTimer:
+    """hipEvent timing on the null stream, where the library launches."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                                 ctypes.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.a, self.b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(ctypes.byref(self.b)) == 0
+
+    def sync(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+    def start(self):
+        self.sync()
+        assert self.hip.hipEventRecord(self.a, None) == 0
+
+    def stop(self):
+        assert self.hip.hipEventRecord(self.b, None) == 0
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        self.sync()
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), self.a, self.b) == 0
+        return ms.value
+
+
+def unit_spins(rng, shape):
+    v = rng.normal(size=shape)
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def mc_workload():
+    c = H.Couplings(J1=1.0, J2=0.12, JX=0.05, h=0.3, lambda1=0.9, lambda2=0.9)
+    lat = H.Lattice.generate("triang", 96, c)
+    temps = np.geomspace(0.02, 2.0, 256)
+    return c, lat, temps
+
+
+def stripe_noise(rng, lat, sigma=0.15):
+    """Horizontal-stripe state (spins along z, HStpPhase) with Gaussian angular
+    noise: each spin is rotated about a random axis perpendicular to it."""
+    L = lat.L
+    n = lat.N
+    s = np.zeros((n, 3))
+    j = np.arange(n) % L
+    s[:, 2] = np.cos(np.pi * j)
+    ax = rng.normal(size=(n, 3))
+    ax -= (ax * s).sum(1, keepdims=True) * s
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    ang = rng.normal(scale=sigma, size=(n, 1))
+    s = s * np.cos(ang) + np.cross(ax, s) * np.sin(ang)
+    return s / np.linalg.norm(s, axis=1, keepdims=True)
+
+
+def smd_workload(rng):
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lats = [H.Lattice.generate("square", 64, c), H.Lattice.generate("kagome", 32, c),
+            H.Lattice.generate("hexagn", 40, c), H.Lattice.generate("square", 48, c)]
+    vac = np.zeros(lats[3].N, np.int32)
+    vac[rng.choice(lats[3].N, 230, replace=False)] = 1
+    lats[3].set_vacancies(vac)
+    idx = np.repeat(np.arange(4), 128).astype(np.int32)
+    samples = []
+    for i in idx:
+        if i in (0, 3):
+            samples.append(stripe_noise(rng, lats[i]))
+        else:
+            samples.append(unit_spins(rng, (lats[i].N, 3)))
+    return lats, idx, samples
+
+
+def cpu_baseline(lat, c, temps, seconds_target=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_ref
+    cores = os.cpu_count() or 1
+    R = min(len(temps), max(1, cores))
+    sel = np.linspace(0, len(temps) - 1, R).round().astype(int)
+    m = cpu_ref.Model(lat.tables(), (lat.zn1, lat.zn2, lat.znx), c.as_params())
+    rng = np.random.default_rng(9)
+    s = unit_spins(rng, (R, lat.N, 3))
+    # calibrate with one sweep, then run a bounded sample
+    t0 = time.perf_counter()
+    m.mc_reference(0, temps[sel], s, 1, n_or=5, n_hb=1, n_or_b=5, n_hb_b=1, pt_start=0,
+                   threads=R)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(200, int(seconds_target / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    m.mc_reference(0, temps[sel], s, n, n_or=5, n_hb=1, n_or_b=5, n_hb_b=1, pt_start=0,
+                   threads=R)
+    dt = time.perf_counter() - t0
+    ups = R * lat.N * 6 * n / dt
+    return {"value": ups, "unit": "spin-updates/s", "cores": R, "kind": "port",
+            "sample": "reference-order FP64 sweeps (random permutation, 5 OR + 1 HB, bond-list "
+                      "energy, replica exchange) of %d replicas of the triangular L=96 lattice "
+                      "for %d sweeps, one OpenMP thread per replica" % (R, n)}
+
+
+def read_traffic():
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", "mc_sweep_f32_triang96_r256_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--smd-steps", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-smd", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if H.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    timer = HipTimer()
+    rng = np.random.default_rng(1)
+    c, lat, temps = mc_workload()
+    R, N = len(temps), lat.N
+    ups_per_sweep = R * N * 6
+    print("[bench] MC workload: triang L=96 N=%d R=%d colours=%d" % (N, R, lat.n_colours),
+          file=sys.stderr, flush=True)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, n_or_boost=5,
+                      n_hb_boost=1, pt_cap50=False, seed=3)
+    mc.set_spins(unit_spins(rng, (R, N, 3)))
+    mc.run(args.warmup, pt_parity=2)
+    timer.start()
+    mc.run(args.steps, pt_parity=2)
+    ms = timer.stop()
+    value = ups_per_sweep * args.steps / (ms * 1e-3)
+    # dominant kernel alone: sweeps without the exchange kernel
+    ksteps = max(10, args.steps // 2)
+    timer.start()
+    mc.run(ksteps, pt_parity=-1)
+    kms = timer.stop() / ksteps
+    bytes_per_update = (lat.n_bonds * 2 // N + 2) * 12 + 4 * (lat.n_bonds * 2 // N)
+    z = 18
+    bytes_per_update = (z + 2) * 12 + 4 * z
+    achieved = ups_per_sweep * bytes_per_update / (kms * 1e-3) / 1e9
+    traffic = read_traffic()
+    att, acc = mc.pt_counters()
+    print("[bench] MC batched f32: %.3f ms/sweep, %.3e spin-updates/s, kernel %.3f ms"
+          % (ms / args.steps, value, kms), file=sys.stderr, flush=True)
+    out = {
+        "metric": "MC spin-updates/s & SMD site-RK4-steps/s: batched mixed-prec vs plain FP64",
+        "value": value,
+        "unit": "spin-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 spins/fields, f64 energies and exchange",
+        "data": "synthetic: random unit spins, seeded",
+        "config": {"workload": "batched PT precision form: triangular L=96 N=9216, J1=1 J2=0.12 "
+                               "JX=0.05 h=0.3 lambda=0.9, 256 replicas T in [0.02,2] geometric, "
+                               "sweep = 5 OR + 1 HB + replica exchange",
+                   "replicas": R, "sites": N, "colours": lat.n_colours,
+                   "pt_accept_rate": float(acc.sum() / max(1, att.sum()))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "heis_sweep_kernel<float,true>",
+                     "bytes_per_update": bytes_per_update, "kernel_ms": kms},
+    }
+    del mc
+    if not args.no_plain:
+        mp = H.MonteCarlo(lat, temps, precision=H.PLAIN_F64, n_or=5, n_hb=1, n_or_boost=5,
+                          n_hb_boost=1, pt_cap50=False, seed=3)
+        mp.set_spins(unit_spins(rng, (R, N, 3)))
+        mp.run(1, pt_parity=2)
+        ps = 3
+        timer.start()
+        mp.run(ps, pt_parity=2)
+        pms = timer.stop()
+        pv = ups_per_sweep * ps / (pms * 1e-3)
+        out["plain_f64"] = {"value": pv, "ms_per_step": pms / ps, "steps": ps,
+                            "batched_over_plain": value / pv}
+        print("[bench] MC plain f64: %.1f ms/sweep" % (pms / ps), file=sys.stderr, flush=True)
+        del mp
+    if not args.no_smd:
+        lats, idx, samples = smd_workload(rng)
+        sites = sum(lats[i].N for i in idx)
+        T = 8192
+        smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32)
+        smd.set_samples(samples)
+        smd.step(16)  # warm-up (records t = 0 and 16 steps)
+        k = args.smd_steps
+        timer.start()
+        smd.step(k)
+        sms = timer.stop()
+        srate = sites * k / (sms * 1e-3)
+        out["smd"] = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
+                      "ms_per_step": sms / k, "samples": len(idx), "sites": sites,
+                      "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
+                                  "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
+                                  "correction, DSSF time contraction over 512 bins included"}
+        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s" % (sms / k, srate),
+              file=sys.stderr, flush=True)
+        del smd
+        c2 = H.Couplings(J1=1.0, J2=0.6)
+        l2 = H.Lattice.generate("square", 24, c2)
+        s2 = [stripe_noise(rng, l2) for _ in range(32)]
+        sp = H.SpinDynamics([l2], np.zeros(32, np.int32), 0.01, 4096, n_omega=512,
+                            precision=H.PLAIN_F64)
+        sp.set_samples(s2)
+        sp.step(4)
+        k2 = 32
+        timer.start()
+        sp.step(k2)
+        pms = timer.stop()
+        prate = 32 * l2.N * k2 / (pms * 1e-3)
+        sb = H.SpinDynamics([l2], np.zeros(32, np.int32), 0.01, 4096, n_omega=512,
+                            precision=H.BATCHED_F32)
+        sb.set_samples(s2)
+        sb.step(4)
+        timer.start()
+        sb.step(256)
+        bms = timer.stop()
+        brate = 32 * l2.N * 256 / (bms * 1e-3)
+        out["smd"]["plain_f64_square24_s32"] = {"value": prate, "ms_per_step": pms / k2}
+        out["smd"]["batched_f32_square24_s32"] = {"value": brate, "ms_per_step": bms / 256,
+                                                  "batched_over_plain": brate / prate}
+        print("[bench] SMD square24: plain f64 %.3e, batched f32 %.3e" % (prate, brate),
+              file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(lat, c, temps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

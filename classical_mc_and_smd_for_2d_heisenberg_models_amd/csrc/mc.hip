@@ -41,6 +41,7 @@ struct SweepArgs {
   int n_colours;
   int col_off[kCMAX + 1];
   int n_or, n_hb, n_or_b, n_hb_b;
+  int metro_passes;        // Ising: Metropolis passes per sweep (1; 0 = energies only)
   uint64_t seed;
   uint32_t sweep;
   int config_base;
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a
   const double beta = a.beta_slot[slot];
   const double *Jt = (const double *)a.Jt;
   const uint32_t step = a.sweep * 16u;
-  for (int col = 0; col < a.n_colours; col++) {
+  for (int col = 0; col < (a.metro_passes > 0 ? a.n_colours : 0); col++) {
     for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
       double h = a.cp.h_on ? -a.cp.h : 0.0;
       for (int k = 0; k < a.cp.Z; k++) {
@@ -616,6 +617,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   a.n_colours = lat.n_colours;
   for (int k = 0; k <= lat.n_colours; k++) a.col_off[k] = lat.col_off[k];
   a.n_or = sp.n_or; a.n_hb = sp.n_hb; a.n_or_b = sp.n_or_boost; a.n_hb_b = sp.n_hb_boost;
+  a.metro_passes = 1;
   a.seed = sp.seed;
   a.n_stag = lat.labels.n_stag;
   a.has_labels = lat.labels.present ? 1 : 0;
@@ -755,13 +757,19 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
   p.moments = e.moments.p; p.n_meas = e.n_meas.p;
   e.args.measure = measure;
   const int pthreads = ((e.R + 63) / 64) * 64;
+  // the measurement / exchange kernel is skipped when it has nothing to do
+  const bool post = pt_parity >= 0 || measure || etrace_out;
   for (int s = 0; s < n_sweeps; s++) {
     e.args.sweep = e.sweep;
     launch_sweep(e, 0);
-    p.sweep = e.sweep;
-    p.parity = pt_parity == 2 ? (int)(e.sweep % 2u) : pt_parity;
-    p.etrace = etrace_out ? e.etrace.p + (size_t)s * e.R : nullptr;
-    hipLaunchKernelGGL(measure_exchange_kernel, dim3(1), dim3(pthreads), 0, 0, p);
+    if (post) {
+      p.sweep = e.sweep;
+      p.parity = pt_parity == 2 ? (int)(e.sweep % 2u) : pt_parity;
+      p.etrace = etrace_out ? e.etrace.p + (size_t)s * e.R : nullptr;
+      hipLaunchKernelGGL(measure_exchange_kernel, dim3(1), dim3(pthreads), 0, 0, p);
+    } else if (s == 0) {
+      e.boost.zero();
+    }
     e.sweep++;
   }
   HIP_CHECK(hipGetLastError());
@@ -783,6 +791,7 @@ void mc_refresh_energies(MCEngine &e) {
   // zero-pass sweep: recompute energies of the current configurations
   SweepArgs a = e.args;
   a.n_or = a.n_hb = a.n_or_b = a.n_hb_b = 0;
+  a.metro_passes = 0;
   a.measure = 0;
   std::swap(a, e.args);
   try {

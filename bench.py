@@ -156,6 +156,28 @@ def read_traffic():
     return None
 
 
+def calibrate_copy(timer, nbytes=1 << 30, reps=5):
+    """Device-to-device copies of a known size: the HBM counters of this blit
+    (FETCH_SIZE = WRITE_SIZE = 1 GiB per copy ideally) calibrate the counters."""
+    hip = timer.hip
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), nbytes) == 0
+    assert hip.hipMalloc(ctypes.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    timer.start()
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    ms = timer.stop()
+    print("[bench] copy %d x %d bytes: %.3f ms each, %.1f GB/s (read+write)"
+          % (reps, nbytes, ms / reps, 2 * nbytes * reps / (ms * 1e-3) / 1e9), file=sys.stderr)
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -174,6 +198,9 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     timer = HipTimer()
+    if args.calibrate_copy:
+        calibrate_copy(timer)
+        return
     rng = np.random.default_rng(1)
     c, lat, temps = mc_workload()
     R, N = len(temps), lat.N
@@ -193,9 +220,8 @@ def main():
     timer.start()
     mc.run(ksteps, pt_parity=-1)
     kms = timer.stop() / ksteps
-    bytes_per_update = (lat.n_bonds * 2 // N + 2) * 12 + 4 * (lat.n_bonds * 2 // N)
-    z = 18
-    bytes_per_update = (z + 2) * 12 + 4 * z
+    z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
+    bytes_per_update = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
     achieved = ups_per_sweep * bytes_per_update / (kms * 1e-3) / 1e9
     traffic = read_traffic()
     att, acc = mc.pt_counters()

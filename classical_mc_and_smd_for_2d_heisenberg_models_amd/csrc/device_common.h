@@ -58,7 +58,38 @@ struct Coup {
   double jxy[kZMAX];         // uniform couplings per slot
   double jz[kZMAX];          // uniform z couplings per slot (J * lambda)
   double lam[kZMAX];         // lambda per slot (general mode)
+  float fjxy[kZMAX];         // single-precision copies for the FP32 kernels
+  float fjz[kZMAX];
+  float flam[kZMAX];
 };
+
+template <typename T>
+__device__ __forceinline__ T cj_xy(const Coup &cp, int s);
+template <>
+__device__ __forceinline__ double cj_xy<double>(const Coup &cp, int s) { return cp.jxy[s]; }
+template <>
+__device__ __forceinline__ float cj_xy<float>(const Coup &cp, int s) { return cp.fjxy[s]; }
+template <typename T>
+__device__ __forceinline__ T cj_z(const Coup &cp, int s);
+template <>
+__device__ __forceinline__ double cj_z<double>(const Coup &cp, int s) { return cp.jz[s]; }
+template <>
+__device__ __forceinline__ float cj_z<float>(const Coup &cp, int s) { return cp.fjz[s]; }
+template <typename T>
+__device__ __forceinline__ T cj_lam(const Coup &cp, int s);
+template <>
+__device__ __forceinline__ double cj_lam<double>(const Coup &cp, int s) { return cp.lam[s]; }
+template <>
+__device__ __forceinline__ float cj_lam<float>(const Coup &cp, int s) { return cp.flam[s]; }
+
+// Fills the single-precision copies of the slot couplings.
+inline void finish_coup(Coup &cp) {
+  for (int s = 0; s < kZMAX; s++) {
+    cp.fjxy[s] = (float)cp.jxy[s];
+    cp.fjz[s] = (float)cp.jz[s];
+    cp.flam[s] = (float)cp.lam[s];
+  }
+}
 
 struct Bonds {
   int Nb;
@@ -68,35 +99,57 @@ struct Bonds {
 };
 
 // h_i of the reference's get_localField (CF/subrouts.cpp:4465-4575):
-// start from -h on z, then add shells in slot order.
-template <typename T>
+// start from -h on z, then add shells in slot order.  ZC is a compile-time
+// bound on the slot count, so every neighbour index is loaded up front and
+// the gathers overlap instead of waiting on one index at a time.
+template <typename T, int ZC>
 __device__ __forceinline__ void local_field(const Coup &cp, const int32_t *__restrict__ nb,
                                             const T *__restrict__ Jt, int p,
                                             const T *sx, const T *sy, const T *sz,
                                             T &hx, T &hy, T &hz) {
+  const int N = cp.N, Z = cp.Z;
+  int q[ZC];
+#pragma unroll
+  for (int s = 0; s < ZC; s++) q[s] = (s < Z) ? nb[(size_t)s * N + p] : N;
   hx = T(0);
   hy = T(0);
   hz = cp.h_on ? T(-cp.h) : T(0);
-  const int N = cp.N;
   if (!cp.general) {
-    for (int s = 0; s < cp.Z; s++) {
-      const int q = nb[(size_t)s * N + p];
-      const T J = T(cp.jxy[s]), Jz = T(cp.jz[s]);
-      hx += J * sx[q];
-      hy += J * sy[q];
-      hz += Jz * sz[q];
+#pragma unroll
+    for (int s = 0; s < ZC; s++) {
+      if (s < Z) {
+        const T J = cj_xy<T>(cp, s), Jz = cj_z<T>(cp, s);
+        hx += J * sx[q[s]];
+        hy += J * sy[q[s]];
+        hz += Jz * sz[q[s]];
+      }
     }
   } else {
-    for (int s = 0; s < cp.Z; s++) {
-      const int q = nb[(size_t)s * N + p];
-      const T J = Jt[(size_t)s * N + p];
-      const T Jz = J * T(cp.lam[s]);
-      hx += J * sx[q];
-      hy += J * sy[q];
-      hz += Jz * sz[q];
+#pragma unroll
+    for (int s = 0; s < ZC; s++) {
+      if (s < Z) {
+        const T J = Jt[(size_t)s * N + p];
+        const T Jz = J * cj_lam<T>(cp, s);
+        hx += J * sx[q[s]];
+        hy += J * sy[q[s]];
+        hz += Jz * sz[q[s]];
+      }
     }
   }
 }
+
+// Compile-time slot bound for a lattice with Z active slots.
+inline int zc_for(int Z) { return Z <= 8 ? 8 : Z <= 16 ? 16 : Z <= 20 ? 20 : 32; }
+
+#define HSMD_ZC_DISPATCH(Z, FN, ...)             \
+  do {                                           \
+    switch (::hsmd::zc_for(Z)) {                 \
+      case 8: FN<8>(__VA_ARGS__); break;         \
+      case 16: FN<16>(__VA_ARGS__); break;       \
+      case 20: FN<20>(__VA_ARGS__); break;       \
+      default: FN<32>(__VA_ARGS__); break;       \
+    }                                            \
+  } while (0)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -48,6 +48,7 @@ struct SweepArgs {
   int measure;
   int n_stag, has_labels, z2, c3;
   const int32_t *nb;
+  const uint32_t *nb16;    // packed 16-bit neighbour pairs, (Z+1)/2 x N (fast path)
   const void *Jt;
   const int32_t *perm;
   const int32_t *bi, *bj;
@@ -170,6 +171,24 @@ __device__ __forceinline__ void heat_bath<float>(double beta, float hx, float hy
 
 // ------------------------------------------------------- sweep epilogues
 
+// Adds (x, y, z) to the sublattice sums selected by the labels; every index
+// is a compile-time constant so the sums stay in registers.
+__device__ __forceinline__ void add_labels(double *v, int ls, int lh, int lv, int ld, double x,
+                                           double y, double z) {
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double w = (ls == q) ? 1.0 : 0.0;
+    v[3 + 3 * q] += w * x; v[4 + 3 * q] += w * y; v[5 + 3 * q] += w * z;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const double wh = (lh == q) ? 1.0 : 0.0, wv = (lv == q) ? 1.0 : 0.0, wd = (ld == q) ? 1.0 : 0.0;
+    v[12 + 3 * q] += wh * x; v[13 + 3 * q] += wh * y; v[14 + 3 * q] += wh * z;
+    v[18 + 3 * q] += wv * x; v[19 + 3 * q] += wv * y; v[20 + 3 * q] += wv * z;
+    v[24 + 3 * q] += wd * x; v[25 + 3 * q] += wd * y; v[26 + 3 * q] += wd * z;
+  }
+}
+
 // FP64 bond-list energy (get_energyValue) and order-parameter sums.
 template <typename T>
 __device__ void heis_epilogue(const SweepArgs &a, int c, const T *sx, const T *sy, const T *sz,
@@ -181,44 +200,68 @@ __device__ void heis_epilogue(const SweepArgs &a, int c, const T *sx, const T *s
     e += a.bd.jxy[k] * ((double)sx[i] * (double)sx[j] + (double)sy[i] * (double)sy[j]) +
          a.bd.jz[k] * ((double)sz[i] * (double)sz[j]);
   }
+  if (a.cp.h_on) {
+    double mz = 0.0;
+    for (int p = threadIdx.x; p < N; p += blockDim.x) mz += (double)sz[p];
+    e -= a.cp.h * mz;
+  }
+  double v1[1] = {e};
+  block_sum<1>(v1, scratch);
+  if (threadIdx.x == 0) a.energy[c] = v1[0];
+}
+
+// Order-parameter sums of one configuration per workgroup, read back from
+// global memory after the sweep (measurement sweeps only): total M, the
+// staggered / stripe sublattice sums and the Z2 plaquette sum.
+template <typename T>
+__device__ __forceinline__ void load_spin(const T *g, int Npad, int p, double &x, double &y,
+                                          double &z) {
+  x = (double)g[p]; y = (double)g[Npad + p]; z = (double)g[2 * Npad + p];
+}
+template <>
+__device__ __forceinline__ void load_spin<int8_t>(const int8_t *g, int Npad, int p, double &x,
+                                                  double &y, double &z) {
+  x = (double)g[p]; y = 0.0; z = 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
+  __shared__ double scratch[NOBS * 16];
+  const int c = blockIdx.x;
+  const int N = a.cp.N, Npad = a.cp.Npad;
+  const int comps = sizeof(T) == 1 ? 1 : 3;
+  const T *g = (const T *)a.spins + (size_t)c * comps * Npad;
   double v[NOBS];
 #pragma unroll
   for (int k = 0; k < NOBS; k++) v[k] = 0.0;
-  double mz = 0.0;
   for (int p = threadIdx.x; p < N; p += blockDim.x) {
-    const double x = sx[p], y = sy[p], z = sz[p];
-    mz += z;
-    if (a.measure) {
-      v[0] += x; v[1] += y; v[2] += z;
-      if (a.has_labels) {
-        const int ls = a.lab[p], lh = a.lab[N + p], lv = a.lab[2 * N + p], ld = a.lab[3 * N + p];
-        if (ls >= 0) { v[3 + 3 * ls] += x; v[4 + 3 * ls] += y; v[5 + 3 * ls] += z; }
-        if (lh >= 0) { v[12 + 3 * lh] += x; v[13 + 3 * lh] += y; v[14 + 3 * lh] += z; }
-        if (lv >= 0) { v[18 + 3 * lv] += x; v[19 + 3 * lv] += y; v[20 + 3 * lv] += z; }
-        if (ld >= 0) { v[24 + 3 * ld] += x; v[25 + 3 * ld] += y; v[26 + 3 * ld] += z; }
-        if (a.z2) {
-          const int qx = a.z2nb[p], qy = a.z2nb[N + p], qxy = a.z2nb[2 * N + p];
-          const double d = (x - (double)sx[qxy]) * ((double)sx[qx] - (double)sx[qy]) +
-                           (y - (double)sy[qxy]) * ((double)sy[qx] - (double)sy[qy]) +
-                           (z - (double)sz[qxy]) * ((double)sz[qx] - (double)sz[qy]);
-          const double fc = (fabs(d) > 1.0e-12) ? 1.0 / fabs(d) : 0.25;
-          v[30] += fc * d;
-        }
+    double x, y, z;
+    load_spin<T>(g, Npad, p, x, y, z);
+    v[0] += x; v[1] += y; v[2] += z;
+    if (a.has_labels) {
+      add_labels(v, a.lab[p], a.lab[N + p], a.lab[2 * N + p], a.lab[3 * N + p], x, y, z);
+      if (a.z2) {
+        double ax, ay, az, bx, by, bz, cx, cy, cz;
+        load_spin<T>(g, Npad, a.z2nb[p], ax, ay, az);
+        load_spin<T>(g, Npad, a.z2nb[N + p], bx, by, bz);
+        load_spin<T>(g, Npad, a.z2nb[2 * N + p], cx, cy, cz);
+        // IsingZ2Parameter: [S(i) - S(i+x+y)] . [S(i+x) - S(i+y)]
+        const double d = (x - cx) * (ax - bx) + (y - cy) * (ay - by) + (z - cz) * (az - bz);
+        const double fc = (fabs(d) > 1.0e-12) ? 1.0 / fabs(d) : 0.25;
+        v[30] += fc * d;
       }
     }
   }
-  v[31] = e - (a.cp.h_on ? a.cp.h * mz : 0.0);
   block_sum<NOBS>(v, scratch);
   if (threadIdx.x == 0) {
-    a.energy[c] = v[31];
-    if (a.measure)
-      for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
+#pragma unroll
+    for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
   }
 }
 
 // ----------------------------------------------------------- Heisenberg
 
-template <typename T, bool LDS>
+template <typename T, bool LDS, int ZC>
 __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBS * 16];
@@ -246,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a)
     for (int col = 0; col < a.n_colours; col++) {
       for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
         T hx, hy, hz;
-        local_field<T>(a.cp, a.nb, Jt, p, sx, sy, sz, hx, hy, hz);
+        local_field<T, ZC>(a.cp, a.nb, Jt, p, sx, sy, sz, hx, hy, hz);
         T x = sx[p], y = sy[p], z = sz[p];
         if (!hb) {
           overrelax<T>(hx, hy, hz, x, y, z);
@@ -265,12 +308,88 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a)
   }
 }
 
+
+// Fast path of the sweep: LDS-resident replica, uniform couplings, every
+// colour class no larger than the workgroup (one site per thread per class),
+// neighbour labels packed as 16-bit pairs.  The labels of the next class are
+// fetched before the current class is updated, so their L2 latency overlaps
+// the gathers, the update and the class barrier.
+template <typename T, int ZC>
+__global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[NOBS * 16];
+  constexpr int ZH = (ZC + 1) / 2;
+  const int c = blockIdx.x;
+  const int N = a.cp.N, Npad = a.cp.Npad, Z = a.cp.Z;
+  T *g = (T *)a.spins + (size_t)c * 3 * Npad;
+  T *s = (T *)smem;
+  for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) s[i] = g[i];
+  T *sx = s, *sy = s + Npad, *sz = s + 2 * Npad;
+  const int slot = a.slot_of_config[c];
+  const double beta = a.beta_slot[slot];
+  const bool boost = a.boost_slot[slot] != 0;
+  const int nor = boost ? a.n_or_b : a.n_or;
+  const int nhb = boost ? a.n_hb_b : a.n_hb;
+  const int nc = a.n_colours;
+  const int total = (nor + nhb) * nc;
+  const int tid = threadIdx.x;
+  const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
+  const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
+  uint32_t qa[ZH], qb[ZH];
+  int la = 0, lb = 0;
+  auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab) {
+    const int p = a.col_off[col] + tid;
+    const bool v = p < a.col_off[col + 1];
+#pragma unroll
+    for (int k = 0; k < ZH; k++) q[k] = (v && 2 * k < Z) ? a.nb16[(size_t)k * N + p] : pad;
+    lab = v ? a.perm[p] : 0;
+  };
+  if (total > 0) fetch(0, qa, la);
+  __syncthreads();
+  for (int it = 0; it < total; it++) {
+    const int pass = it / nc;
+    const int col = it - pass * nc;
+    if (it + 1 < total) fetch((it + 1) % nc, qb, lb);
+    const int p = a.col_off[col] + tid;
+    if (p < a.col_off[col + 1]) {
+      T hx = T(0), hy = T(0), hz = hz0;
+#pragma unroll
+      for (int k = 0; k < ZH; k++) {
+        const int i0 = (int)(qa[k] & 0xffffu), i1 = (int)(qa[k] >> 16);
+        if (2 * k < Z) {
+          const T J = cj_xy<T>(a.cp, 2 * k), Jz = cj_z<T>(a.cp, 2 * k);
+          hx += J * sx[i0]; hy += J * sy[i0]; hz += Jz * sz[i0];
+        }
+        if (2 * k + 1 < Z) {
+          const T J = cj_xy<T>(a.cp, 2 * k + 1), Jz = cj_z<T>(a.cp, 2 * k + 1);
+          hx += J * sx[i1]; hy += J * sy[i1]; hz += Jz * sz[i1];
+        }
+      }
+      T x = sx[p], y = sy[p], z = sz[p];
+      if (pass < nor) {
+        overrelax<T>(hx, hy, hz, x, y, z);
+      } else {
+        const uint32_t step = a.sweep * 16u + (uint32_t)pass;
+        const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
+        heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
+      }
+      sx[p] = x; sy[p] = y; sz[p] = z;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ZH; k++) qa[k] = qb[k];
+    la = lb;
+  }
+  heis_epilogue<T>(a, c, sx, sy, sz, scratch);
+  for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) g[i] = s[i];
+}
+
 // --------------------------------------------------------------- Ising
 
 // Metropolis with FP64 fields and the deterministic exponential; spins are
 // int8 +-1 with a zero sentinel.  Energies use exact integer bond counts per
 // coupling class, E = sum_c J_c n_c - h M, evaluated in class order.
-template <bool LDS>
+template <bool LDS, int ZC>
 __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBS * 16];
@@ -293,10 +412,15 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a
   for (int col = 0; col < (a.metro_passes > 0 ? a.n_colours : 0); col++) {
     for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
       double h = a.cp.h_on ? -a.cp.h : 0.0;
-      for (int k = 0; k < a.cp.Z; k++) {
-        const int q = a.nb[(size_t)k * N + p];
-        const double J = a.cp.general ? Jt[(size_t)k * N + p] : a.cp.jxy[k];
-        h += J * (double)s[q];
+      int q[ZC];
+#pragma unroll
+      for (int k = 0; k < ZC; k++) q[k] = (k < a.cp.Z) ? a.nb[(size_t)k * N + p] : N;
+#pragma unroll
+      for (int k = 0; k < ZC; k++) {
+        if (k < a.cp.Z) {
+          const double J = a.cp.general ? Jt[(size_t)k * N + p] : a.cp.jxy[k];
+          h += J * (double)s[q[k]];
+        }
       }
       const double spin = (double)s[p];
       const double dE = 2.0 * spin * h;
@@ -323,46 +447,25 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a
 #pragma unroll
     for (int k = 0; k <= kBCMAX; k++) cnt[k] = wave_sum_ll(cnt[k]);
     __syncthreads();
-    if (lane == 0)
+    if (lane == 0) {
+#pragma unroll
       for (int k = 0; k <= kBCMAX; k++) iscr[k * 16 + w] = cnt[k];
+    }
     __syncthreads();
+#pragma unroll
     for (int k = 0; k <= kBCMAX; k++) {
       long long t = 0;
       for (int i = 0; i < nw; i++) t += iscr[k * 16 + i];
       cnt[k] = t;
     }
   }
-  // order parameters on spins (s, 0, 0)
-  double v[NOBS];
-#pragma unroll
-  for (int k = 0; k < NOBS; k++) v[k] = 0.0;
-  if (a.measure) {
-    for (int p = threadIdx.x; p < N; p += blockDim.x) {
-      const double x = s[p];
-      v[0] += x;
-      if (a.has_labels) {
-        const int ls = a.lab[p], lh = a.lab[N + p], lv = a.lab[2 * N + p], ld = a.lab[3 * N + p];
-        if (ls >= 0) v[3 + 3 * ls] += x;
-        if (lh >= 0) v[12 + 3 * lh] += x;
-        if (lv >= 0) v[18 + 3 * lv] += x;
-        if (ld >= 0) v[24 + 3 * ld] += x;
-        if (a.z2) {
-          const int qx = a.z2nb[p], qy = a.z2nb[N + p], qxy = a.z2nb[2 * N + p];
-          const double d = (x - (double)s[qxy]) * ((double)s[qx] - (double)s[qy]);
-          const double fc = (fabs(d) > 1.0e-12) ? 1.0 / fabs(d) : 0.25;
-          v[30] += fc * d;
-        }
-      }
-    }
-  }
-  block_sum<NOBS>(v, scratch);
   if (threadIdx.x == 0) {
     double E = 0.0;
-    for (int k = 0; k < a.bd.n_class; k++) E = fma(a.bd.jxy[k], (double)cnt[k], E);
+#pragma unroll
+    for (int k = 0; k < kBCMAX; k++)
+      if (k < a.bd.n_class) E = fma(a.bd.jxy[k], (double)cnt[k], E);
     if (a.cp.h_on) E = fma(-a.cp.h, (double)cnt[kBCMAX], E);
     a.energy[c] = E;
-    if (a.measure)
-      for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
   }
   if (LDS) {
     for (int i = threadIdx.x; i < Npad; i += blockDim.x) g[i] = s[i];
@@ -447,31 +550,34 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
       if (k == 0) a.n_meas[0] += 1;
     }
   }
+  // the exchange: pairs (k1, k1 + 1) of one parity are disjoint, so every
+  // pair is decided by its own thread
   __syncthreads();
-  if (t == 0) {
-    for (int k = 0; k < a.R; k++) a.boost_slot[k] = 0;
-    if (a.parity >= 0) {
-      for (int k1 = a.parity; k1 + 1 < a.R; k1 += 2) {
-        const int k2 = k1 + 1;
-        a.pt_cnt[k1] += 1;
-        a.pt_cnt[k2] += 1;
-        const double ratio = (double)(a.pt_exc[k2] + 1) * (100.0 / (double)a.pt_cnt[k2]);
-        const int c1 = a.config_of_slot[k1], c2 = a.config_of_slot[k2];
-        const double dE = a.energy[c2] - a.energy[c1];
-        const double dB = a.beta_slot[k2] - a.beta_slot[k1];
-        const double fc = dB * dE;
-        const u32x4 r = draw4(a.seed, (uint32_t)k1, a.sweep, 0u, STREAM_PT);
-        const double u = u01_f64(r.x, r.y);
-        if (accept_exp(u, fc) && (!a.cap50 || ratio <= 50.0)) {
-          a.config_of_slot[k1] = c2;
-          a.config_of_slot[k2] = c1;
-          a.slot_of_config[c2] = k1;
-          a.slot_of_config[c1] = k2;
-          a.pt_exc[k1] += 1;
-          a.pt_exc[k2] += 1;
-          a.boost_slot[k1] = 1;
-          a.boost_slot[k2] = 1;
-        }
+  if (t < a.R) a.boost_slot[t] = 0;
+  __syncthreads();
+  if (a.parity >= 0) {
+    const int k1 = a.parity + 2 * t;
+    if (k1 + 1 < a.R) {
+      const int k2 = k1 + 1;
+      const long long cnt1 = a.pt_cnt[k1] + 1, cnt2 = a.pt_cnt[k2] + 1;
+      a.pt_cnt[k1] = cnt1;
+      a.pt_cnt[k2] = cnt2;
+      const double ratio = (double)(a.pt_exc[k2] + 1) * (100.0 / (double)cnt2);
+      const int c1 = a.config_of_slot[k1], c2 = a.config_of_slot[k2];
+      const double dE = a.energy[c2] - a.energy[c1];
+      const double dB = a.beta_slot[k2] - a.beta_slot[k1];
+      const double fc = dB * dE;
+      const u32x4 r = draw4(a.seed, (uint32_t)k1, a.sweep, 0u, STREAM_PT);
+      const double u = u01_f64(r.x, r.y);
+      if (accept_exp(u, fc) && (!a.cap50 || ratio <= 50.0)) {
+        a.config_of_slot[k1] = c2;
+        a.config_of_slot[k2] = c1;
+        a.slot_of_config[c2] = k1;
+        a.slot_of_config[c1] = k2;
+        a.pt_exc[k1] += 1;
+        a.pt_exc[k2] += 1;
+        a.boost_slot[k1] = 1;
+        a.boost_slot[k2] = 1;
       }
     }
   }
@@ -491,6 +597,8 @@ struct MCEngine {
   bool spins_set = false;
   std::vector<double> temps;
   DevBuf<int32_t> nb, perm, bi, bj, z2nb;
+  DevBuf<uint32_t> nb16;
+  bool fast = false;
   DevBuf<uint8_t> bc;
   DevBuf<int8_t> lab;
   DevBuf<float> Jf;
@@ -516,6 +624,7 @@ static void fill_coup(const Lattice &L, Coup &cp, bool ising) {
     cp.jz[s] = ising ? L.shellJ[sh - 1] : L.shellJ[sh - 1] * L.shellLam[sh - 1];
     cp.lam[s] = L.shellLam[sh - 1];
   }
+  finish_coup(cp);
 }
 
 void upload_lattice_common(const Lattice &L, DevBuf<int32_t> &nb, DevBuf<int32_t> &perm,
@@ -544,6 +653,9 @@ void fill_bonds(const Lattice &L, Bonds &bd) {
 }
 
 static size_t lds_limit() { return 160 * 1024 - NOBS * 16 * 8 - (kBCMAX + 1) * 16 * 8 - 1024; }
+
+template <int ZC>
+static void set_lds_attr(MCEngine &e);
 
 MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp) {
   if (R < 1 || R > kThreads) throw Error(HSMD_ERR_ARG, "replica count must be in [1, 1024]");
@@ -635,13 +747,24 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   a.boost_slot = e->boost.p;
   a.energy = e->energy.p;
   a.obs = e->obs.p;
-  if (e->use_lds) {
-    const void *fn;
-    if (e->ising) fn = (const void *)ising_sweep_kernel<true>;
-    else if (f32) fn = (const void *)heis_sweep_kernel<float, true>;
-    else fn = (const void *)heis_sweep_kernel<double, true>;
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_bytes));
+  {
+    int maxcol = 0;
+    for (int k = 0; k < lat.n_colours; k++) maxcol = std::max(maxcol, lat.col_off[k + 1] - lat.col_off[k]);
+    e->fast = e->use_lds && !e->ising && !lat.general && (lat.N + 1) < 65535 && maxcol <= kThreads;
+    if (e->fast) {
+      const int ZH = (lat.Z + 1) / 2;
+      std::vector<uint32_t> pk((size_t)std::max(ZH, 1) * N);
+      for (int k = 0; k < ZH; k++)
+        for (int p = 0; p < N; p++) {
+          const uint32_t lo = (uint32_t)lat.g_nb[(size_t)(2 * k) * N + p];
+          const uint32_t hi = (2 * k + 1 < lat.Z) ? (uint32_t)lat.g_nb[(size_t)(2 * k + 1) * N + p] : (uint32_t)N;
+          pk[(size_t)k * N + p] = lo | (hi << 16);
+        }
+      e->nb16.upload(pk.data(), pk.size());
+      a.nb16 = e->nb16.p;
+    }
   }
+  if (e->use_lds) HSMD_ZC_DISPATCH(lat.Z, set_lds_attr, *e);
   return e.release();
 }
 
@@ -711,15 +834,16 @@ void mc_get_spins(MCEngine &e, double *spins) {
   }
 }
 
-static void launch_sweep(MCEngine &e, hipStream_t st) {
+template <int ZC>
+static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
   SweepArgs &a = e.args;
   const bool f32 = e.prec == HSMD_BATCHED_F32;
   if (e.prec == HSMD_PLAIN_F64) {
     // one instance per launch, FP64, spins in global memory
     for (int c = 0; c < e.R; c++) {
       a.config_base = c;
-      if (e.ising) hipLaunchKernelGGL((ising_sweep_kernel<false>), dim3(1), dim3(kThreads), 0, st, a);
-      else hipLaunchKernelGGL((heis_sweep_kernel<double, false>), dim3(1), dim3(kThreads), 0, st, a);
+      if (e.ising) hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(1), dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((heis_sweep_kernel<double, false, ZC>), dim3(1), dim3(kThreads), 0, st, a);
     }
     a.config_base = 0;
     return;
@@ -727,15 +851,36 @@ static void launch_sweep(MCEngine &e, hipStream_t st) {
   a.config_base = 0;
   const size_t sh = e.use_lds ? e.lds_bytes : 0;
   if (e.ising) {
-    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((ising_sweep_kernel<false>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
+  } else if (e.fast && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
+    if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
   } else if (f32) {
-    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_kernel<float, false>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<float, false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
   } else {
-    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<double, true>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_kernel<double, false>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<double, true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<double, false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
   }
+}
+
+template <int ZC>
+static void set_lds_attr(MCEngine &e) {
+  const bool f32 = e.prec == HSMD_BATCHED_F32;
+  const void *fn;
+  if (e.ising) fn = (const void *)ising_sweep_kernel<true, ZC>;
+  else if (f32) fn = (const void *)heis_sweep_kernel<float, true, ZC>;
+  else fn = (const void *)heis_sweep_kernel<double, true, ZC>;
+  HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+  if (e.fast) {
+    fn = f32 ? (const void *)heis_sweep_fast<float, ZC> : (const void *)heis_sweep_fast<double, ZC>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+  }
+}
+
+static void launch_sweep(MCEngine &e, hipStream_t st) {
+  HSMD_ZC_DISPATCH(e.lat.Z, launch_sweep_zc, e, st);
 }
 
 void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrace_out) {
@@ -762,6 +907,12 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
   for (int s = 0; s < n_sweeps; s++) {
     e.args.sweep = e.sweep;
     launch_sweep(e, 0);
+    if (measure) {
+      if (e.ising) hipLaunchKernelGGL(obs_kernel<int8_t>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
+      else if (e.prec == HSMD_BATCHED_F32)
+        hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
+      else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
+    }
     if (post) {
       p.sweep = e.sweep;
       p.parity = pt_parity == 2 ? (int)(e.sweep % 2u) : pt_parity;
@@ -845,7 +996,7 @@ void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st) {
 
 // ------------------------------------------------- fields / energies (ABI)
 
-template <typename T>
+template <typename T, int ZC>
 __global__ void fields_kernel(const Coup cp, const int32_t *nb, const T *Jt, const T *spins,
                               T *out) {
   const int c = blockIdx.y;
@@ -853,7 +1004,7 @@ __global__ void fields_kernel(const Coup cp, const int32_t *nb, const T *Jt, con
   if (p >= cp.N) return;
   const T *s = spins + (size_t)c * 3 * cp.Npad;
   T hx, hy, hz;
-  local_field<T>(cp, nb, Jt, p, s, s + cp.Npad, s + 2 * cp.Npad, hx, hy, hz);
+  local_field<T, ZC>(cp, nb, Jt, p, s, s + cp.Npad, s + 2 * cp.Npad, hx, hy, hz);
   T *o = out + (size_t)c * 3 * cp.N;
   o[p] = hx; o[cp.N + p] = hy; o[2 * cp.N + p] = hz;
 }
@@ -877,6 +1028,12 @@ __global__ __launch_bounds__(kThreads) void energy_kernel(const Coup cp, const B
   double v[1] = {e - (cp.h_on ? cp.h * mz : 0.0)};
   block_sum<1>(v, scratch);
   if (threadIdx.x == 0) E[c] = v[0];
+}
+
+template <int ZC, typename T>
+static void launch_fields(dim3 grid, const Coup &cp, const int32_t *nb, const T *Jt, const T *s,
+                          T *out) {
+  hipLaunchKernelGGL((fields_kernel<T, ZC>), grid, dim3(256), 0, 0, cp, nb, Jt, s, out);
 }
 
 template <typename T>
@@ -904,8 +1061,7 @@ static void fields_energies_impl(const Lattice &L, int R, const double *spins, d
   if (fields) {
     DevBuf<T> out;
     out.alloc((size_t)3 * N * R);
-    hipLaunchKernelGGL(fields_kernel<T>, dim3((N + 255) / 256, R), dim3(256), 0, 0, cp, nb.p, Jt,
-                       ds.p, out.p);
+    HSMD_ZC_DISPATCH(L.Z, launch_fields, dim3((N + 255) / 256, R), cp, nb.p, Jt, ds.p, out.p);
     HIP_CHECK(hipGetLastError());
     std::vector<T> o((size_t)3 * N * R);
     HIP_CHECK(hipMemcpy(o.data(), out.p, o.size() * sizeof(T), hipMemcpyDeviceToHost));

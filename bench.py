@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--smd-steps", type=int, default=256)
+    ap.add_argument("--smd-steps", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -271,18 +271,37 @@ def main():
         T = 8192
         smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32)
         smd.set_samples(samples)
-        smd.step(16)  # warm-up (records t = 0 and 16 steps)
+        smd.step(1023)  # warm-up: time points 0..1023, i.e. one full trajectory chunk
         k = args.smd_steps
         timer.start()
-        smd.step(k)
+        smd.step(k)     # k RK4 steps; every 1024 points the DSSF chunk GEMM runs
         sms = timer.stop()
         srate = sites * k / (sms * 1e-3)
+        # the chunk GEMM alone: fill a chunk to one short of full, then time the
+        # last step (one RK4 step + the flush) against a plain step
+        rem = 1024 - ((1024 + k) % 1024) - 1
+        if rem > 0:
+            smd.step(rem)
+        timer.start()
+        smd.step(1)
+        t_flush = timer.stop()
+        timer.start()
+        smd.step(1)
+        t_one = timer.stop()
+        gemm_ms = max(t_flush - t_one, 1e-6)
+        gm = sum(3 * lats[i].N for i in idx)
+        mfma_flop = 2.0 * gm * 1024 * 1024 * 3      # M x (2 n_omega) x K, three split products
         out["smd"] = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
                       "ms_per_step": sms / k, "samples": len(idx), "sites": sites,
                       "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
                                   "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
-                                  "correction, DSSF time contraction over 512 bins included"}
-        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s" % (sms / k, srate),
+                                  "correction, DSSF time contraction over 512 bins included",
+                      "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": 1024,
+                                          "mfma_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,
+                                          "peak_tflops_fp16_dense": 2500.0,
+                                          "mfma_util": mfma_flop / (gemm_ms * 1e-3) / 2.5e15}}
+        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s, chunk GEMM %.2f ms "
+              "(%.0f TFLOP/s)" % (sms / k, srate, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
               file=sys.stderr, flush=True)
         del smd
         c2 = H.Couplings(J1=1.0, J2=0.6)

@@ -1,0 +1,221 @@
+// DSSF time contraction on the matrix cores: split-precision FP16 MFMA with
+// FP32 accumulation.
+//
+// The SMD engine accumulates, per sample, Y[m][j] += sum_k A[k][m] B[k][j]
+// where A is a chunk of the trajectory (time k, spin component x site m,
+// FP32) and B the windowed DFT twiddles (time k, frequency bin j with cos in
+// the first n_omega columns and -sin in the rest).  Each FP32 operand is split
+// into two FP16 halves, x = x_hi + x_lo with x_hi = fp16(x) and
+// x_lo = fp16(x - x_hi), and the product is formed as
+//   hi*hi + hi*lo + lo*hi     (the lo*lo term is below FP32 rounding)
+// with three v_mfma_f32_16x16x16_f16 per 16x16x16 block and FP32 accumulation,
+// which keeps about 21 significant bits -- FP32-level accuracy for operands in
+// [-1, 1].  This replaces the reference's per-q MKL FFT over time
+// (CF/subrouts.cpp:12062-12105); the DFT is exact for every bin kept.
+//
+// Tiling: a workgroup of 4 waves owns a 64 (m) x 64 (j) tile of Y; each wave
+// a 32 x 32 quarter as 2 x 2 MFMA blocks.  Per K step of 16 the workgroup
+// stages the FP32 A and B tiles, split into FP16 hi/lo, in LDS with k
+// contiguous so that every lane reads its 4-element fragment as one 8-byte
+// load.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "hsmd_internal.h"
+
+namespace hsmd {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct GemmSample {
+  int M;                 // rows of Y (3 N)
+  long long a_off;       // into A (chunk, [K][M])
+  long long y_off;       // into Y ([M][ldb])
+};
+
+constexpr int TM = 64, TJ = 64, TK = 16, KP = TK + 4;  // KP: padded k stride (halves)
+
+__global__ __launch_bounds__(256) void time_gemm_mfma_kernel(const GemmSample *samples,
+                                                             const float *__restrict__ A,
+                                                             const float *__restrict__ Bm, int ldb,
+                                                             int k0, int K, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[TM * KP], Al[TM * KP];
+  __shared__ __align__(16) _Float16 Bh[TJ * KP], Bl[TJ * KP];
+  const GemmSample sd = samples[blockIdx.z];
+  const int M = sd.M;
+  const int m0 = blockIdx.x * TM, j0 = blockIdx.y * TJ;
+  if (m0 >= M) return;
+  const float *Ab = A + sd.a_off;
+  float *Yb = Y + sd.y_off;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave >> 1) * 32, wj = (wave & 1) * 32;   // wave's quarter
+  const int li = lane & 15, lk = (lane >> 4) * 4;           // fragment coordinates
+  float4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  for (int kk = 0; kk < K; kk += TK) {
+    // stage A (16 x 64) and B (16 x 64): 1024 elements each, 4 per thread
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int e = tid + r * 256;
+      const int k = e >> 6, col = e & 63;
+      const int kg = kk + k;
+      const int m = m0 + col;
+      const float av = (kg < K && m < M) ? Ab[(size_t)kg * M + m] : 0.f;
+      const float bv = (kg < K) ? Bm[(size_t)(k0 + kg) * ldb + j0 + col] : 0.f;
+      const _Float16 ah = (_Float16)av;
+      const _Float16 bh = (_Float16)bv;
+      Ah[col * KP + k] = ah;
+      Al[col * KP + k] = (_Float16)(av - (float)ah);
+      Bh[col * KP + k] = bh;
+      Bl[col * KP + k] = (_Float16)(bv - (float)bh);
+    }
+    __syncthreads();
+    half4_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int row = wm + a * 16 + li;
+      ah[a] = *(const half4_t *)&Ah[row * KP + lk];
+      al[a] = *(const half4_t *)&Al[row * KP + lk];
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int col = wj + b * 16 + li;
+      bh[b] = *(const half4_t *)&Bh[col * KP + lk];
+      bl[b] = *(const half4_t *)&Bl[col * KP + lk];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  // D layout of the 16x16 block: lane holds rows 4*(lane/16) + r, column lane % 16
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = j0 + wj + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
+        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+      }
+    }
+}
+
+// Host launcher: samples[S] on the device, Y accumulated in place.
+void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float *A, const float *B,
+                           int ldb, int k0, int K, float *Y, hipStream_t st) {
+  if (ldb % TJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 64");
+  dim3 grid((maxM + TM - 1) / TM, ldb / TJ, S);
+  hipLaunchKernelGGL(time_gemm_mfma_kernel, grid, dim3(256), 0, st,
+                     (const GemmSample *)samples_dev, A, B, ldb, k0, K, Y);
+}
+
+
+// Wide-tile variant for 2 n_omega a multiple of 256: a workgroup of 8 waves
+// owns a 128 (m) x 256 (j) tile, so each trajectory element is staged once
+// per 256 frequency columns and each twiddle once per 128 rows; the twiddles
+// arrive pre-split (FP16 hi/lo, k contiguous per column) and the gfx950
+// v_mfma_f32_16x16x32_f16 consumes 32 time points per instruction.  Each wave
+// owns a 64 x 64 sub-tile (4 x 4 MFMA blocks).
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+constexpr int WM = 128, WJ = 256, WK = 32, WKP = WK + 8, WT = 512;
+
+__global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
+    const GemmSample *samples, const float *__restrict__ A, const _Float16 *__restrict__ Bth,
+    const _Float16 *__restrict__ Btl, int ldt, int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[WM * WKP], Al[WM * WKP];
+  __shared__ __align__(16) _Float16 Bh[WJ * WKP], Bl[WJ * WKP];
+  const GemmSample sd = samples[blockIdx.z];
+  const int M = sd.M;
+  const int m0 = blockIdx.x * WM, j0 = blockIdx.y * WJ;
+  if (m0 >= M) return;
+  const float *Ab = A + sd.a_off;
+  float *Yb = Y + sd.y_off;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave >> 2) * 64, wj = (wave & 3) * 64;
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  float4_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  for (int kk = 0; kk < K; kk += WK) {
+    // A: 32 (k) x 128 (m) FP32 -> hi/lo halves, [m][k]
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int e = tid + r * WT;
+      const int k = e >> 7, col = e & 127;
+      const int kg = kk + k, m = m0 + col;
+      const float av = (kg < K && m < M) ? Ab[(size_t)kg * M + m] : 0.f;
+      const _Float16 ah = (_Float16)av;
+      Ah[col * WKP + k] = ah;
+      Al[col * WKP + k] = (_Float16)(av - (float)ah);
+    }
+    // B: 256 columns x 32 time points, half a column (16 points) per thread
+    {
+      const int col = tid >> 1, half = tid & 1;
+      const size_t src = (size_t)(j0 + col) * ldt + (size_t)(k0 + kk + 16 * half);
+      const uint4 *h = (const uint4 *)(Bth + src);
+      const uint4 *l = (const uint4 *)(Btl + src);
+      uint4 *dh = (uint4 *)&Bh[col * WKP + 16 * half];
+      uint4 *dl = (uint4 *)&Bl[col * WKP + 16 * half];
+      dh[0] = h[0]; dh[1] = h[1];
+      dl[0] = l[0]; dl[1] = l[1];
+    }
+    __syncthreads();
+    half8_t ah[4], al[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int row = wm + a * 16 + li;
+      ah[a] = *(const half8_t *)&Ah[row * WKP + lk];
+      al[a] = *(const half8_t *)&Al[row * WKP + lk];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int col = wj + b * 16 + li;
+      const half8_t bh = *(const half8_t *)&Bh[col * WKP + lk];
+      const half8_t bl = *(const half8_t *)&Bl[col * WKP + lk];
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int j = j0 + wj + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
+        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+      }
+    }
+}
+
+void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
+                                const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
+                                int ldb, float *Y, hipStream_t st) {
+  if (ldb % WJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 256");
+  dim3 grid((maxM + WM - 1) / WM, ldb / WJ, S);
+  hipLaunchKernelGGL(time_gemm_mfma_wide_kernel, grid, dim3(WT), 0, st,
+                     (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
+}
+
+}  // namespace hsmd

@@ -152,6 +152,19 @@ __global__ void dft_cols_kernel(int n_batch, int G, int n_omega, const T *R, con
 
 // ---------------------------------------------------------------- engine
 
+void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float *A, const float *B,
+                           int ldb, int k0, int K, float *Y, hipStream_t st);
+
+void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
+                                const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
+                                int ldb, float *Y, hipStream_t st);
+
+struct GemmSampleHost {
+  int M;
+  long long a_off;
+  long long y_off;
+};
+
 struct SMDEngine {
   std::vector<Lattice> lats;
   hsmd_smd_spec spec{};
@@ -180,6 +193,9 @@ struct SMDEngine {
   DevBuf<unsigned char> spins, s0, stage, traj, ybuf, twid, dftcs;
   DevBuf<double> qss, eref, dssf;
   DevBuf<int> sample_ids;
+  DevBuf<GemmSampleHost> gemm_samples;
+  DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 n_omega][ldt]
+  int ldt = 0;
   bool set = false;
 };
 
@@ -199,6 +215,21 @@ static void upload_twiddles(SMDEngine &e) {
   }
   e.twid.alloc(B.size() * sizeof(T));
   HIP_CHECK(hipMemcpy(e.twid.p, B.data(), B.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (sizeof(T) == 4 && (2 * no) % 256 == 0) {
+    // transposed FP16 split of the twiddles for the wide MFMA contraction;
+    // rows padded to a multiple of 32 time points with zeros
+    e.ldt = ((T_ + 31) / 32) * 32;
+    std::vector<_Float16> hi((size_t)2 * no * e.ldt, (_Float16)0.f), lo(hi.size(), (_Float16)0.f);
+    for (int t = 0; t < T_; t++)
+      for (int j = 0; j < 2 * no; j++) {
+        const float v = (float)B[(size_t)t * 2 * no + j];
+        const _Float16 h = (_Float16)v;
+        hi[(size_t)j * e.ldt + t] = h;
+        lo[(size_t)j * e.ldt + t] = (_Float16)(v - (float)h);
+      }
+    e.tw_hi.upload(hi.data(), hi.size());
+    e.tw_lo.upload(lo.data(), lo.size());
+  }
 }
 
 SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
@@ -278,7 +309,9 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->dlats.upload(hl.data(), hl.size());
   // sizes and layout
   const int T_ = sp.n_steps;
-  e->Tc = std::min(T_, 256);
+  // trajectory chunk: long enough that the Y accumulators are revisited
+  // rarely, short enough to stay a few GB at the largest batches
+  e->Tc = std::min(T_, e->f32 ? 1024 : 256);
   const int no = sp.n_omega;
   long long so = 0, to = 0, yo = 0;
   e->hs.resize(S);
@@ -290,6 +323,11 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
     yo += 3LL * N * 2 * no;
   }
   e->dsamples.upload(e->hs.data(), S);
+  {
+    std::vector<GemmSampleHost> gs(S);
+    for (int s = 0; s < S; s++) gs[s] = {3 * e->hs[s].N, e->hs[s].traj_off, e->hs[s].y_off};
+    e->gemm_samples.upload(gs.data(), S);
+  }
   e->spins.alloc(so * es);
   e->spins.zero();
   e->s0.alloc(so * es);
@@ -392,10 +430,14 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.f32)
-    hipLaunchKernelGGL(time_gemm_kernel<float>, grid, dim3(256), 0, 0, e.dsamples.p,
-                       (const float *)e.traj.p, (const float *)e.twid.p, ldb, e.chunk_base,
-                       e.chunk_fill, (float *)e.ybuf.p);
+  if (e.f32 && e.ldt > 0)
+    launch_time_gemm_mfma_wide(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw_hi.p,
+                               e.tw_lo.p, e.ldt, e.chunk_base, e.chunk_fill, ldb,
+                               (float *)e.ybuf.p, 0);
+  else if (e.f32)
+    launch_time_gemm_mfma(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p,
+                          (const float *)e.twid.p, ldb, e.chunk_base, e.chunk_fill,
+                          (float *)e.ybuf.p, 0);
   else
     hipLaunchKernelGGL(time_gemm_kernel<double>, grid, dim3(256), 0, 0, e.dsamples.p,
                        (const double *)e.traj.p, (const double *)e.twid.p, ldb, e.chunk_base,

@@ -100,12 +100,14 @@ def test_dssf_fp64_matches_oracle(geom, L):
     np.testing.assert_allclose(q, qref, rtol=1e-10)
 
 
-def test_dssf_fp32_within_1e3_of_fp64():
-    _say("dssf fp32 vs fp64")
+@pytest.mark.parametrize("no", [64, 128])
+def test_dssf_fp32_within_1e3_of_fp64(no):
+    # no = 64 runs the 64-wide MFMA tiles, no = 128 the 256-wide gfx950 tiles
+    _say("dssf fp32 vs fp64", no)
     c = H.Couplings(J1=1.0, J2=0.6)
     lat = H.Lattice.generate("square", 8, c)
     rng = np.random.default_rng(8)
-    S, T, no = 4, 256, 64
+    S, T = 4, 300
     samples = [_spins(rng, lat.N) for _ in range(S)]
     out = {}
     for prec in (H.BATCHED_F64, H.BATCHED_F32):

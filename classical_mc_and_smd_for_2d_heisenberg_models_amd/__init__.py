@@ -104,6 +104,8 @@ SIGNATURES = {
     "hsmd_smd_dssf": (_I, [_P, _I, _D, _D]),
     "hsmd_smd_energies": (_I, [_P, _D]),
     "hsmd_smd_destroy": (None, [_P]),
+    "hsmd_run_mc": (_I, [ctypes.c_char_p, _I, _I, ctypes.c_uint64, _I]),
+    "hsmd_run_smd": (_I, [ctypes.c_char_p, _I, _I]),
     "hsmd_debug_philox": (_I, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, _I, _U32]),
     "hsmd_debug_exp": (_I, [_I, _D, _D]),
@@ -371,6 +373,17 @@ def get_dynSpectrum(lattice, samples, dt, n_steps, window=None, n_omega=None,
     smd.step(n_steps - 1)
     d, q = smd.dssf(0)
     return d / S, q / S
+
+
+def run_mc(workdir, n_replicas, precision=BATCHED_F32, seed=1, strict=True):
+    """The reference's MC program (xMC_run) on the Settings_*.txt files of workdir."""
+    _check(lib().hsmd_run_mc(os.fsencode(workdir), int(n_replicas), int(precision), int(seed),
+                             int(strict)))
+
+
+def run_smd(workdir, precision=BATCHED_F32, strict=True):
+    """The reference's SMD program (xSMD_run) on the Settings_*.txt files of workdir."""
+    _check(lib().hsmd_run_smd(os.fsencode(workdir), int(precision), int(strict)))
 
 
 def debug_philox(seed, step, inst, stream, n):

@@ -46,6 +46,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, OUT)
+    # the drop-in executables (xMC_run / xSMD_run) over the C ABI
+    for exe in ("xMC_run", "xSMD_run"):
+        dst = os.path.join(HERE, exe)
+        cmd = ["g++", "-O2", "-std=c++17", os.path.join(CSRC, "driver_main.cpp"), "-o", dst + ".tmp",
+               "-L" + HERE, "-lhsmd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("driver link failed: %s\n%s" % (r.stdout, r.stderr))
+        os.replace(dst + ".tmp", dst)
     if verbose:
         print("built", OUT)
     return OUT

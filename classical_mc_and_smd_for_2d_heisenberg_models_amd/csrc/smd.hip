@@ -194,19 +194,21 @@ struct SMDEngine {
   DevBuf<double> qss, eref, dssf;
   DevBuf<int> sample_ids;
   DevBuf<GemmSampleHost> gemm_samples;
-  DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 n_omega][ldt]
+  DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 nop][ldt]
+  int nop = 0;                     // frequency bins held internally (n_omega padded)
   int ldt = 0;
   bool set = false;
 };
 
 template <typename T>
 static void upload_twiddles(SMDEngine &e) {
-  const int T_ = e.spec.n_steps, no = e.spec.n_omega;
-  std::vector<T> B((size_t)T_ * 2 * no);
+  // bins n >= n_omega of the padded table are zero columns
+  const int T_ = e.spec.n_steps, no = e.nop, nkeep = e.spec.n_omega;
+  std::vector<T> B((size_t)T_ * 2 * no, T(0));
   const double tp = 2.0 * acos(-1.0);
   for (int t = 0; t < T_; t++) {
     const double w = e.window[t];
-    for (int n = 0; n < no; n++) {
+    for (int n = 0; n < nkeep; n++) {
       const long long ph = ((long long)n * t) % T_;
       const double th = tp * (double)ph / (double)T_;
       B[(size_t)t * 2 * no + n] = (T)(w * cos(th));
@@ -312,7 +314,8 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // trajectory chunk: long enough that the Y accumulators are revisited
   // rarely, short enough to stay a few GB at the largest batches
   e->Tc = std::min(T_, e->f32 ? 1024 : 256);
-  const int no = sp.n_omega;
+  e->nop = e->f32 ? ((sp.n_omega + 127) / 128) * 128 : ((sp.n_omega + 31) / 32) * 32;
+  const int no = e->nop;
   long long so = 0, to = 0, yo = 0;
   e->hs.resize(S);
   for (int s = 0; s < S; s++) {
@@ -424,7 +427,7 @@ void smd_launch_f32(const SmdLaunch &c, StepArgs a) {
 
 static void flush_chunk(SMDEngine &e) {
   if (!e.spec.dssf || e.chunk_fill == 0) return;
-  const int no = e.spec.n_omega;
+  const int no = e.nop;
   int maxM = 0;
   for (auto &s : e.hs) maxM = std::max(maxM, 3 * s.N);
   const int ldb = 2 * no;
@@ -486,7 +489,7 @@ void smd_step(SMDEngine &e, int n) {
 template <typename T>
 static void dssf_space(SMDEngine &e, int li, double *out) {
   const Lattice &L = e.lats[li];
-  const int G = L.Gsz, no = e.spec.n_omega, Nq = G * G;
+  const int G = L.Gsz, no = e.nop, nkeep = e.spec.n_omega, Nq = G * G;
   std::vector<int> ids;
   for (int s = 0; s < e.S; s++) if (e.sample_lat[s] == li) ids.push_back(s);
   DevBuf<double> d;
@@ -512,7 +515,10 @@ static void dssf_space(SMDEngine &e, int li, double *out) {
                        dsn.p, scale, d.p);
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipMemcpy(out, d.p, (size_t)3 * Nq * no * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> h((size_t)3 * Nq * no);
+  HIP_CHECK(hipMemcpy(h.data(), d.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t dq = 0; dq < (size_t)3 * Nq; dq++)
+    for (int n = 0; n < nkeep; n++) out[dq * nkeep + n] = h[dq * no + n];
 }
 
 void smd_dssf(SMDEngine &e, int li, double *out, double *qss) {

@@ -204,6 +204,23 @@ hsmd_status hsmd_smd_dssf(hsmd_smd *smd, int li, double *out, double *qss);
 hsmd_status hsmd_smd_energies(hsmd_smd *smd, double *energies);
 void hsmd_smd_destroy(hsmd_smd *smd);
 
+/* ----------------------------------------------------------- drop-in runs */
+
+/* Full MC run in workdir (replaces main of CF/mc_hm_code.cpp:139, launched
+ * there as `mpirun --np n_replicas ./xMC_run`): reads the five Settings_*.txt
+ * files, builds the lattice (writing outBin/nborsList.bin), thermalises,
+ * measures in bins of 1000 sweeps with replica exchange, and appends
+ * outBin/MC_Bin_AvgValue{1,2,3}_T(..)<label>.bin and
+ * outBin/MC_Bin_SpinField_T(..)<label>.bin, then writes
+ * outBin/Temp_List<label>.bin.  strict = 1 keeps every reference guard rail. */
+hsmd_status hsmd_run_mc(const char *workdir, int n_replicas, int precision, uint64_t seed,
+                        int strict);
+
+/* Full SMD run in workdir (replaces main of CF/smd_code.cpp:141): evolves
+ * every sample of outBin/MC_Bin_SpinField_T(TpVec[tpIndex])<label>.bin and
+ * writes the sample-averaged outBin/MC_Bin_SpecData_T(..)<label>.bin. */
+hsmd_status hsmd_run_smd(const char *workdir, int precision, int strict);
+
 /* --------------------------------------------------------- self checks */
 
 /* Writes n Philox4x32-10 blocks for counters (site = i, step, inst, stream)

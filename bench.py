@@ -146,7 +146,6 @@ def read_traffic():
     p = os.path.join(ROOT, "profiles", "mc_sweep_f32_triang96_r256_pmc.txt")
     if not os.path.exists(p):
         return None
-    vals = {}
     for line in open(p):
         if line.startswith("traffic_bytes_per_launch"):
             try:
@@ -247,7 +246,7 @@ def main():
                    "pt_accept_rate": float(acc.sum() / max(1, att.sum()))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "heis_sweep_kernel<float,true>",
+                     "kernel": "heis_sweep_fast<float,20>",
                      "bytes_per_update": bytes_per_update, "kernel_ms": kms},
     }
     del mc

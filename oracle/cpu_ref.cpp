@@ -769,6 +769,107 @@ void orc_dssf(int N, int G, const int32_t *gridmap, int T, const double *traj,
     }
 }
 
+
+// --- order parameters (CF/subrouts.cpp:5362-5800) ---------------------------
+// geom: "square", "triang", "hexagn", "kagome", "lieb"; spins N x 3.
+// out[8]: |M|/N, M_z/N (abs without a field), absStaggMag, absStripeMag
+// Horz / Vert / Diag, SMag1 = sqrt(H^2 + V^2 + D^2 [C3]), and the
+// cplxC3Parameter (C3 geometries) or IsingZ2Parameter (square).
+void orc_observables(const char *geom, int L, int N, const int32_t *nbors, int zn1, int zn2,
+                     int znx, int h_on, const double *S, double *out) {
+  const std::string g(geom);
+  const bool c3 = (g != "square" && g != "lieb");
+  const int Zt = zn1 + zn2 + znx;
+  const double iNs = 1.0 / N;
+  double M[3] = {0, 0, 0};
+  for (int k = 0; k < N; k++) for (int d = 0; d < 3; d++) M[d] += S[3 * k + d];
+  out[0] = iNs * sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+  out[1] = M[2] * iNs;
+  if (!h_on) out[1] = fabs(out[1]);
+  auto finish = [&](double sub[][3], int n0, double norm) {
+    double sv[3] = {0, 0, 0}, sq = 0;
+    for (int n = 0; n < n0; n++) {
+      double mv[3];
+      for (int d = 0; d < 3; d++) { mv[d] = norm * sub[n][d]; sv[d] += mv[d]; }
+      sq += mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2];
+    }
+    return (1.0 / sqrt((double)n0)) * sqrt(sq - (sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]) / n0);
+  };
+  // absStaggMag
+  {
+    const int n0 = c3 ? 3 : 2, ifc = c3 ? 1 : 0;
+    double sub[3][3] = {{0}};
+    if (g != "lieb") {
+      int k = 0;
+      for (int i = 0; i < L; i++) {
+        for (int j = 0; j < L; j++) {
+          const int st = i * L + j, n = (i + j + ifc * k) % n0;
+          for (int d = 0; d < 3; d++) sub[n][d] += S[3 * st + d];
+        }
+        k += 1;
+      }
+    } else {
+      for (int k = 0; k < N; k++) for (int d = 0; d < 3; d++) sub[k % n0][d] += S[3 * k + d];
+    }
+    out[2] = finish(sub, n0, n0 * iNs);
+  }
+  // absStripeMag
+  const char *types[3] = {"Horz", "Vert", "Diag"};
+  for (int t = 0; t < 3; t++) {
+    double sub[2][3] = {{0}};
+    const std::string ty(types[t]);
+    if (g != "lieb") {
+      for (int i = 0; i < L; i++)
+        for (int j = 0; j < L; j++) {
+          const int st = i * L + j;
+          const int n = ty == "Horz" ? j % 2 : ty == "Vert" ? i % 2 : (i + j) % 2;
+          for (int d = 0; d < 3; d++) sub[n][d] += S[3 * st + d];
+        }
+    } else if (ty == "Horz" || ty == "Vert") {
+      int n = 0;
+      for (int i = 0; i < L; i++)
+        for (int j = 0; j < L; j++) {
+          const int n1 = (i + j) % 2;
+          for (int d = 0; d < 3; d++) {
+            if (ty == "Horz") sub[(i + j) & 1][d] += S[3 * n + d] + S[3 * (n + 1) + d];
+            else { sub[n1][d] += S[3 * n + d]; sub[1 - n1][d] += S[3 * (n + 1) + d]; }
+          }
+          n += 2;
+        }
+    } else {
+      for (int k = 0; k < N; k++) for (int d = 0; d < 3; d++) sub[k % 2][d] += S[3 * k + d];
+    }
+    out[3 + t] = finish(sub, 2, 2.0 * iNs);
+  }
+  out[6] = sqrt(out[3] * out[3] + out[4] * out[4] + (c3 ? out[5] * out[5] : 0.0));
+  out[7] = 0.0;
+  if (c3) {
+    // cplxC3Parameter, z component (dMag = 2)
+    double sm[3] = {0, 0, 0};
+    int k = 0;
+    for (int i = 0; i < L; i++) {
+      for (int j = 0; j < L; j++) sm[(i + j + k) % 3] += S[3 * (i * L + j) + 2];
+      k += 1;
+    }
+    const double re = sqrt(6.0) * iNs * (sm[0] + sm[1] - 2.0 * sm[2]);
+    const double im = -3.0 * sqrt(2.0) * iNs * (sm[1] - sm[0]);
+    out[7] = sqrt(re * re + im * im);
+  } else if (g == "square") {
+    // IsingZ2Parameter: [S(i) - S(i+x+y)] . [S(i+x) - S(i+y)], sign per plaquette
+    double sum = 0.0;
+    for (int i = 0; i < N; i++) {
+      const int nxx = nbors[(size_t)i * Zt + 0], nyy = nbors[(size_t)i * Zt + 1];
+      const int nxy = nbors[(size_t)i * Zt + zn1 + 0];
+      double dp = 0.0;
+      for (int d = 0; d < 3; d++)
+        dp += (S[3 * i + d] - S[3 * nxy + d]) * (S[3 * nxx + d] - S[3 * nyy + d]);
+      const double fc = fabs(dp) > 1.0e-12 ? 1.0 / fabs(dp) : 0.25;
+      sum += fc * dp;
+    }
+    out[7] = iNs * sum;
+  }
+}
+
 // --- windows (CF/aux_code.cpp:249-435, selection CF/smd_code.cpp:2825-2895) -
 int orc_window(const char *name, int N, double *w) {
   const std::string s(name);

@@ -41,6 +41,7 @@ _SIG = {
     "orc_smd_run": (None, [_P, _D, ctypes.c_double, _I, _I, _D, _D, _D]),
     "orc_dssf": (None, [_I, _I, _I32, _I, _D, _D, _I, _D]),
     "orc_window": (_I, [ctypes.c_char_p, _I, _D]),
+    "orc_observables": (None, [ctypes.c_char_p, _I, _I, _I32, _I, _I, _I, _I, _D, _D]),
 }
 
 _lib = None
@@ -218,3 +219,13 @@ def heat_bath(beta, h, u1, u2):
     o = np.zeros(3)
     lib().orc_heat_bath(beta, _dp(h), u1, u2, _dp(o))
     return o
+
+
+def observables(geom, L, nbors, zn, h_on, spins):
+    """[|M|/N, Mz/N, stagg, stripe H, V, D, SMag1, C3-Psi or Z2] of one configuration."""
+    nb = np.ascontiguousarray(nbors, dtype=np.int32)
+    s = np.ascontiguousarray(spins, dtype=np.float64)
+    out = np.zeros(8)
+    lib().orc_observables(geom.encode(), L, nb.shape[0], _ip(nb), zn[0], zn[1], zn[2], int(h_on),
+                          _dp(s), _dp(out))
+    return out

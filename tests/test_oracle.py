@@ -370,3 +370,35 @@ def test_windows_match_textbook_definitions():
     np.testing.assert_allclose(cpu_ref.window("sincw", n), np.sinc(2 * x), atol=1e-14)
     lz = np.where(np.abs(0.5 * x) > 0.5, 0.0, np.sinc(x))
     np.testing.assert_allclose(cpu_ref.window("lancz", n), lz, atol=1e-14)
+
+
+def test_order_parameters_on_ordered_states():
+    # perfect states give the textbook values of the reference's comments
+    L = 6
+    nb, rv, gm, zn = cpu_ref.lattice("square", L)
+    ii, jj = np.meshgrid(np.arange(L), np.arange(L), indexing="ij")
+    neel = np.zeros((L * L, 3)); neel[:, 2] = np.cos(np.pi * (ii + jj)).ravel()
+    o = cpu_ref.observables("square", L, nb, zn, 0, neel)
+    assert o[0] == pytest.approx(0, abs=1e-14)
+    assert o[2] == pytest.approx(1.0, abs=1e-14)            # staggered magnetisation
+    assert o[5] == pytest.approx(1.0, abs=1e-14)            # diagonal stripe = Neel on square
+    hst = np.zeros((L * L, 3)); hst[:, 2] = np.cos(np.pi * jj).ravel()
+    o = cpu_ref.observables("square", L, nb, zn, 0, hst)
+    assert o[3] == pytest.approx(1.0, abs=1e-14) and o[4] == pytest.approx(0.0, abs=1e-7)
+    assert abs(o[7]) == pytest.approx(1.0, abs=1e-14)       # Z2 parameter +-1 on stripes
+    ferro = np.zeros((L * L, 3)); ferro[:, 0] = 1.0
+    o = cpu_ref.observables("square", L, nb, zn, 0, ferro)
+    assert o[0] == pytest.approx(1.0) and o[2] == pytest.approx(0.0, abs=1e-7)
+    # triangular: 120 degree state -> Psi = sqrt(6) with spins in the (z, y) plane
+    nb, rv, gm, zn = cpu_ref.lattice("triang", L)
+    th = (4 * np.pi / 3) * (ii + 0.5 * jj)
+    s = np.zeros((L * L, 3)); s[:, 2] = np.cos(th).ravel(); s[:, 1] = np.sin(th).ravel()
+    o = cpu_ref.observables("triang", L, nb, zn, 1, s)
+    assert o[7] == pytest.approx(np.sqrt(6.0), rel=1e-12)
+    assert o[2] == pytest.approx(1.0, rel=1e-12)
+    # 2-up 1-down collinear state (make_2U1DPhaseList): Psi = 8 / sqrt(6), the
+    # value quoted in the reference's cplxC3Parameter comment
+    n = (2 * ii + jj + 2) % 3
+    s = np.zeros((L * L, 3)); s[:, 2] = np.cos(n * np.pi).ravel()
+    o = cpu_ref.observables("triang", L, nb, zn, 1, s)
+    assert o[7] == pytest.approx(8 / np.sqrt(6.0), rel=1e-12)

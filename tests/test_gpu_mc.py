@@ -199,3 +199,46 @@ def test_pt_moves_configurations_and_respects_cap():
     # energies stay ordered on average with temperature
     E = mc.run(200, pt_parity=2, trace=True).mean(axis=0)
     assert np.all(np.diff(E) > -2.0)
+
+
+@pytest.mark.parametrize("geom,L,kw,model", [
+    ("square", 8, dict(J1=1.0, J2=0.6), H.HEISENBERG),
+    ("triang", 6, dict(J1=1.0, h=0.4), H.HEISENBERG),
+    ("lieb", 4, dict(J1=1.0, J2=0.3), H.HEISENBERG),
+    ("square", 8, dict(J1=1.0, J2=0.6), H.ISING),
+])
+def test_measured_order_parameters_match_oracle(geom, L, kw, model):
+    # one measured sweep: the per-slot moments equal the oracle's order
+    # parameters (absStaggMag, absStripeMag, IsingZ2Parameter / cplxC3Parameter,
+    # |M|, M_z) of the configuration after the sweep
+    _say("observables", geom, L, model)
+    c = H.Couplings(**kw)
+    lat = H.Lattice.generate(geom, L, c)
+    temps = np.array([0.3, 0.8, 1.5, 3.0])
+    rng = np.random.default_rng(13)
+    if model == H.ISING:
+        s = np.zeros((4, lat.N, 3)); s[:, :, 0] = rng.choice([-1.0, 1.0], size=(4, lat.N))
+    else:
+        s = _spins(rng, 4, lat.N)
+    mc = H.MonteCarlo(lat, temps, model=model, precision=H.BATCHED_F64, n_or=1, n_hb=1, seed=2)
+    mc.set_spins(s)
+    mc.run(3, pt_parity=2)
+    mc.moments(reset=True)
+    mc.run(1, pt_parity=-1, measure=True)
+    mom, n = mc.moments()
+    assert n == 1
+    sp = mc.get_spins()
+    E = mc.energies()
+    nb = lat.tables()
+    zn = (lat.zn1, lat.zn2, lat.znx)
+    for k in range(4):
+        o = cpu_ref.observables(geom, L, nb, zn, int(lat.flags["h_on"]), sp[k])
+        F1, T1, Q1, SH, SV, SD, S1, X1 = o
+        m = mom[k]
+        np.testing.assert_allclose(m[0], E[k], rtol=1e-12)
+        np.testing.assert_allclose([m[2], m[5], m[7], m[13], m[14], m[15], m[10]],
+                                   [F1, T1, Q1, SH, SV, SD, S1], rtol=1e-10, atol=1e-12)
+        x = abs(X1) if geom == "square" else X1
+        np.testing.assert_allclose(m[16], x, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose([m[3], m[8], m[11], m[17]], [F1 ** 2, Q1 ** 2, S1 ** 2, X1 ** 2],
+                                   rtol=1e-10, atol=1e-12)

@@ -75,6 +75,10 @@ struct Lattice {
   std::vector<int32_t> g_perm;          // new -> original (copy of perm)
 
   void rebuild();
+  // True when the bond-list energy equals 1/2 sum_i S_i.h_i - 1/2 h sum_i S_z,i
+  // for every configuration (symmetric tables, each pair in one shell, and
+  // field couplings equal to bond couplings); checked on a test configuration.
+  bool field_energy_ok() const;
   double bondJ(int shell, int i, int j) const;   // J of a bond (energy)
   double slotJ(int shell, int k, int i, int nb) const;  // J of a field slot
   double lam(int shell) const { return shell == 1 ? (xxz_on ? c.lambda1 : 1.0)

@@ -547,4 +547,40 @@ void Lattice::rebuild() {
   }
 }
 
+bool Lattice::field_energy_ok() const {
+  if (general) return false;
+  // a fixed pseudo-random unit-vector configuration
+  std::vector<double> S((size_t)3 * N);
+  uint32_t st = 0x9e3779b9u;
+  auto rnd = [&]() { st ^= st << 13; st ^= st >> 17; st ^= st << 5; return (st >> 8) * (1.0 / 16777216.0) - 0.5; };
+  for (int i = 0; i < N; i++) {
+    double v[3] = {rnd(), rnd(), rnd()};
+    const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 1e-12;
+    for (int d = 0; d < 3; d++) S[3 * i + d] = v[d] / n;
+  }
+  double eb = 0.0, ef = 0.0, sz = 0.0;
+  for (size_t b = 0; b < bi.size(); b++) {
+    const double J = bondJ(bshell[b], bi[b], bj[b]), Jz = J * lam(bshell[b]);
+    const int i = bi[b], j = bj[b];
+    eb += J * (S[3 * i] * S[3 * j] + S[3 * i + 1] * S[3 * j + 1]) + Jz * S[3 * i + 2] * S[3 * j + 2];
+  }
+  for (int p = 0; p < N; p++) {
+    const int i = perm[p];
+    double h[3] = {0, 0, 0};
+    for (int s = 0; s < Z; s++) {
+      const int q = g_nb[(size_t)s * N + p];
+      if (q >= N) continue;
+      const int j = perm[q];
+      const int sh = slot_shell[s];
+      h[0] += shellJ[sh - 1] * S[3 * j];
+      h[1] += shellJ[sh - 1] * S[3 * j + 1];
+      h[2] += shellJ[sh - 1] * shellLam[sh - 1] * S[3 * j + 2];
+    }
+    ef += 0.5 * (S[3 * i] * h[0] + S[3 * i + 1] * h[1] + S[3 * i + 2] * h[2]);
+    sz += S[3 * i + 2];
+  }
+  (void)sz;
+  return fabs(eb - ef) <= 1e-9 * (1.0 + fabs(eb));
+}
+
 }  // namespace hsmd

@@ -173,6 +173,7 @@ struct SMDEngine {
   bool use_lds = false;
   int spt = 1;
   int maxZ = 0;
+  bool general = false;
   size_t lds_bytes = 0;
   int lds_npad = 0;
   int Tc = 0;
@@ -185,6 +186,7 @@ struct SMDEngine {
   std::vector<double> window;
   // device
   std::vector<DevBuf<int32_t>> nb, bi, bj, gmap;
+  std::vector<DevBuf<uint32_t>> nb16;
   std::vector<DevBuf<uint8_t>> bc;
   std::vector<DevBuf<float>> Jf;
   std::vector<DevBuf<double>> Jd;
@@ -258,13 +260,14 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->window.assign(sp.n_steps, 1.0);
   if (window) e->window.assign(window, window + sp.n_steps);
   // per-lattice device tables
-  e->nb.resize(nl); e->bi.resize(nl); e->bj.resize(nl); e->bc.resize(nl);
+  e->nb.resize(nl); e->bi.resize(nl); e->bj.resize(nl); e->bc.resize(nl); e->nb16.resize(nl);
   e->Jf.resize(nl); e->Jd.resize(nl); e->gmap.resize(nl);
   std::vector<LatDev> hl(nl);
   for (int i = 0; i < nl; i++) {
     const Lattice &L = e->lats[i];
     if (L.Z > kZMAX) throw Error(HSMD_ERR_ARG, "too many neighbour slots");
     e->maxZ = std::max(e->maxZ, L.Z);
+    e->general = e->general || L.general;
     e->nb[i].upload(L.g_nb.data(), L.g_nb.size());
     e->bi[i].upload(L.g_bi.data(), L.g_bi.size());
     e->bj[i].upload(L.g_bj.data(), L.g_bj.size());
@@ -296,6 +299,21 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
       }
     }
     d.nb = e->nb[i].p; d.bi = e->bi[i].p; d.bj = e->bj[i].p; d.bc = e->bc[i].p;
+    // 16-bit neighbour labels, two slots per word, kept in registers by the kernel
+    if (L.N + 1 >= 65535) throw Error(HSMD_ERR_ARG, "SMD lattices are limited to 65534 sites");
+    {
+      const int ZH = std::max(1, (L.Z + 1) / 2);
+      std::vector<uint32_t> pk((size_t)ZH * L.N, (uint32_t)L.N | ((uint32_t)L.N << 16));
+      for (int k = 0; 2 * k < L.Z; k++)
+        for (int p = 0; p < L.N; p++) {
+          const uint32_t lo = (uint32_t)L.g_nb[(size_t)(2 * k) * L.N + p];
+          const uint32_t hi = (2 * k + 1 < L.Z) ? (uint32_t)L.g_nb[(size_t)(2 * k + 1) * L.N + p] : (uint32_t)L.N;
+          pk[(size_t)k * L.N + p] = lo | (hi << 16);
+        }
+      e->nb16[i].upload(pk.data(), pk.size());
+      d.nb16 = e->nb16[i].p;
+    }
+    d.field_energy = L.field_energy_ok() ? 1 : 0;
     // DFT grid in new labels; vacant sites are masked out of the transform
     if (!L.gridMap.empty()) {
       std::vector<int32_t> g(L.gridMap.size());
@@ -349,10 +367,11 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   if (e->spt > 8 || (!e->f32 && e->spt > 4))
     throw Error(HSMD_ERR_ARG, "lattice too large for the SMD kernel (N > 8192 FP32, N > 4096 FP64)");
   e->lds_npad = maxN + 1;
-  e->lds_bytes = (size_t)2 * 3 * e->lds_npad * es;
-  e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024;
+  e->lds_bytes = (size_t)3 * 3 * e->lds_npad * es;   // S, B1, B2
+  e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
+               e->spt * (int)es <= 16;
   if (!e->use_lds) {
-    e->stage.alloc(so * es);
+    e->stage.alloc(2 * so * es);
     e->stage.zero();
   }
   return e.release();
@@ -415,16 +434,6 @@ void smd_get_samples(SMDEngine &e, double *spins) {
   }
 }
 
-void smd_launch_f32(const SmdLaunch &c, StepArgs a) {
-  // FP32 state is only used in the batched, LDS-resident form
-  switch (c.spt) {
-    case 1: smd_launch_zc<float, 1>(c, a); break;
-    case 2: smd_launch_zc<float, 2>(c, a); break;
-    case 4: smd_launch_zc<float, 4>(c, a); break;
-    default: smd_launch_zc<float, 8>(c, a); break;
-  }
-}
-
 static void flush_chunk(SMDEngine &e) {
   if (!e.spec.dssf || e.chunk_fill == 0) return;
   const int no = e.nop;
@@ -475,7 +484,8 @@ void smd_step(SMDEngine &e, int n) {
     a.T = e.spec.n_steps;
     a.lds_npad = e.lds_npad;
     a.stage = e.stage.p;
-    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.use_lds, e.lds_bytes};
+    a.stage_stride = (long long)(e.stage.n / (2 * (e.f32 ? sizeof(float) : sizeof(double))));
+    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.general, e.use_lds, e.lds_bytes};
     if (e.f32) smd_launch_f32(lc, a); else smd_launch_f64(lc, a);
     HIP_CHECK(hipGetLastError());
     e.t_next += np;

@@ -13,9 +13,11 @@ struct LatDev {
   Coup cp;
   Bonds bd;
   const int32_t *nb;
+  const uint32_t *nb16;   // packed 16-bit neighbour pairs, (Z+1)/2 x N
   const void *Jt;
   const int32_t *bi, *bj;
   const uint8_t *bc;
+  int field_energy;       // 1: bond energy == 1/2 sum S.h - 1/2 h sum Sz (symmetric tables)
 };
 
 struct SampleDev {
@@ -62,7 +64,8 @@ struct StepArgs {
   void *traj;           // per sample [Tc][3][N]
   double *qss;          // per sample [T]
   double *eref;         // per sample, energy at the current time
-  void *stage;          // per sample [3][Npad] stage buffer (global-memory form)
+  void *stage;          // per sample [3][Npad] stage buffers B1, B2 (global-memory form)
+  long long stage_stride;   // elements from B1 to B2
   double dt;
   int t0;               // global time index of the first point of this launch
   int n_points;         // time points produced by this launch
@@ -72,186 +75,341 @@ struct StepArgs {
   int sample_base;      // first sample of this launch
 };
 
+// Per-lattice constants the time loop reads.  The kernel copies them out of
+// the lattice table into registers before it stores anything, which lets the
+// compiler keep them in scalar registers; read through the table after a
+// store, every coupling would be a vector load waited on in the inner loop.
+template <typename T, int ZC>
+struct FieldCtx {
+  int Z, N;
+  bool general;
+  T hz0;           // -h (field along z) or 0
+  const T *Jt;     // per-slot J table (general lattices)
+  T jxy[ZC];       // uniform couplings per slot
+  T jz[ZC];        // J * lambda per slot, or lambda alone for general lattices
+};
+
+template <typename T, int ZC>
+__device__ __forceinline__ FieldCtx<T, ZC> field_ctx(const Coup &cp, const T *Jt) {
+  FieldCtx<T, ZC> c;
+  c.Z = cp.Z;
+  c.N = cp.N;
+  c.general = cp.general != 0;
+  c.hz0 = cp.h_on ? T(-cp.h) : T(0);
+  c.Jt = Jt;
+#pragma unroll
+  for (int s = 0; s < ZC; s++) {
+    c.jxy[s] = cj_xy<T>(cp, s);
+    c.jz[s] = c.general ? cj_lam<T>(cp, s) : cj_z<T>(cp, s);
+  }
+  return c;
+}
+
+// Local field of site p from its packed 16-bit neighbour labels.  Each label
+// word passes through an empty asm statement so that the compiler unpacks it
+// here instead of hoisting Z x 6 loop-invariant LDS addresses out of the time
+// loop (which spills them to scratch).  Without per-slot J tables (GEN false)
+// all ZC slots are summed with no branch: slots beyond the lattice's Z hold
+// the zero sentinel site and zero couplings, and the LDS reads of every slot
+// issue back to back.  GEN lattices (disorder, Lieb J2 asymmetry) read J per
+// site and slot and test the slot count at run time.
+template <typename T, int ZC, int ZH, bool GEN>
+__device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t (&q)[ZH], int p,
+                                        const T *sx, const T *sy, const T *sz, T &hx, T &hy,
+                                        T &hz) {
+  hx = T(0);
+  hy = T(0);
+  hz = c.hz0;
+  if constexpr (!GEN) {
+#pragma unroll
+    for (int k = 0; k < ZH; k++) {
+      uint32_t w = q[k];
+      asm volatile("" : "+v"(w));
+      const int i0 = (int)(w & 0xffffu), i1 = (int)(w >> 16);
+      hx += c.jxy[2 * k] * sx[i0]; hy += c.jxy[2 * k] * sy[i0]; hz += c.jz[2 * k] * sz[i0];
+      if (2 * k + 1 < ZC) {
+        hx += c.jxy[2 * k + 1] * sx[i1]; hy += c.jxy[2 * k + 1] * sy[i1];
+        hz += c.jz[2 * k + 1] * sz[i1];
+      }
+    }
+  } else {
+    const T *jr = c.Jt;
+    if (c.general) {
+      int pp = p;
+      asm volatile("" : "+v"(pp));
+      jr = c.Jt + pp;
+    }
+#pragma unroll
+    for (int k = 0; k < ZH; k++) {
+      uint32_t w = q[k];
+      asm volatile("" : "+v"(w));
+      const int i0 = (int)(w & 0xffffu), i1 = (int)(w >> 16);
+      if (2 * k < c.Z) {
+        const T J = c.general ? jr[(size_t)(2 * k) * c.N] : c.jxy[2 * k];
+        const T Jz = c.general ? J * c.jz[2 * k] : c.jz[2 * k];
+        hx += J * sx[i0]; hy += J * sy[i0]; hz += Jz * sz[i0];
+      }
+      if (2 * k + 1 < ZC && 2 * k + 1 < c.Z) {
+        const T J = c.general ? jr[(size_t)(2 * k + 1) * c.N] : c.jxy[2 * k + 1];
+        const T Jz = c.general ? J * c.jz[2 * k + 1] : c.jz[2 * k + 1];
+        hx += J * sx[i1]; hy += J * sy[i1]; hz += Jz * sz[i1];
+      }
+    }
+  }
+}
+
 // One launch produces n_points time points t0 .. t0+n_points-1 of every
 // sample; t = 0 is the input configuration, every later point is one RK4
-// step plus the energy correction.
-template <typename T, int SPT, bool LDS, int ZC>
+// step plus the energy correction.  Stage configurations alternate between
+// two buffers (B1, B2), so each stage writes its successor's input without a
+// second barrier, and stage 4 writes S in place (it reads only B1 and the
+// site's own S).  Neighbour labels stay in registers when SPT x ZH <= 16
+// words and the t = 0 spins (for qSS) when SPT <= 2; otherwise they are
+// re-read where used.
+// When the tables are symmetric (field_energy), E_ref and E(S) come from the
+// fields the step computes anyway, E = 1/2 sum_i S_i.h_i - 1/2 h sum_i S_z,i;
+// otherwise from the bond list.
+template <typename T, int SPT, bool LDS, int ZC, bool GEN>
 __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double scratch[2 * 16];
+  __shared__ double scratch[4 * 16];
+  constexpr int ZH = (ZC + 1) / 2;
+  constexpr bool QREG = SPT * ZH <= 16;   // labels in registers
+  constexpr bool GREG = SPT <= 2;         // t = 0 spins in registers
   const int sidx = blockIdx.x + a.sample_base;
   const SampleDev sd = a.samples[sidx];
   const LatDev &L = a.lats[sd.li];
   const int N = sd.N, Npad = N + 1;
+  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L.cp, (const T *)L.Jt);
+  const uint32_t *nb16 = L.nb16;
+  const bool fe = L.field_energy != 0;
+  const double hext = L.cp.h_on ? L.cp.h : 0.0;
   T *gS = (T *)a.spins + sd.spin_off;
   const T *g0 = (const T *)a.s0 + sd.spin_off;
-  T *S, *B;
+  T *S, *B1, *B2;
   int stride;
   if (LDS) {
     stride = a.lds_npad;
     S = (T *)smem;
-    B = S + 3 * stride;
+    B1 = S + 3 * stride;
+    B2 = B1 + 3 * stride;
     for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) {
       const int d = i / Npad, p = i - d * Npad;
       S[d * stride + p] = gS[i];
-      B[d * stride + p] = T(0);
+      B1[d * stride + p] = T(0);
+      B2[d * stride + p] = T(0);
     }
   } else {
-    // global-memory form: spins in place, stage buffer beside them
+    // global-memory form: spins in place, two zeroed stage buffers beside them
     stride = Npad;
     S = gS;
-    B = (T *)a.stage + sd.spin_off;
+    B1 = (T *)a.stage + sd.spin_off;
+    B2 = (T *)a.stage + a.stage_stride + sd.spin_off;
+  }
+  const int Z = fc.Z;
+  const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
+  uint32_t q[QREG ? SPT : 1][ZH];
+  T G0[GREG ? SPT : 1][3];
+#pragma unroll
+  for (int u = 0; u < SPT; u++) {
+    const int p = threadIdx.x + u * kSmdThreads;
+    if constexpr (QREG) {
+#pragma unroll
+      for (int k = 0; k < ZH; k++) q[u][k] = (p < N && 2 * k < Z) ? nb16[(size_t)k * N + p] : pad;
+    }
+    if constexpr (GREG) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) G0[u][d] = p < N ? g0[d * Npad + p] : T(0);
+    }
   }
   __syncthreads();
   T *Sx = S, *Sy = S + stride, *Sz = S + 2 * stride;
-  T *Bx = B, *By = B + stride, *Bz = B + 2 * stride;
+  T *Px = B1, *Py = B1 + stride, *Pz = B1 + 2 * stride;
+  T *Qx = B2, *Qy = B2 + stride, *Qz = B2 + 2 * stride;
   const T dt = T(a.dt);
   double Eref = a.eref[sidx];
-  const T *Jt = (const T *)L.Jt;
   T *traj = (T *)a.traj + sd.traj_off;
+  // field of site p (slot u) from configuration (fx, fy, fz)
+  auto field = [&](int u, int p, const T *fx, const T *fy, const T *fz, T &hx, T &hy, T &hz) {
+    uint32_t w[ZH];
+    if constexpr (QREG) {
+#pragma unroll
+      for (int k = 0; k < ZH; k++) w[k] = q[u][k];
+    } else {
+      int pp = p;
+      asm volatile("" : "+v"(pp));
+#pragma unroll
+      for (int k = 0; k < ZH; k++) w[k] = (2 * k < Z) ? nb16[(size_t)k * N + pp] : pad;
+    }
+    field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
+  };
+  auto spin0 = [&](int u, int p, int d) -> double {
+    if constexpr (GREG) return (double)G0[u][d];
+    else return (double)g0[d * Npad + p];
+  };
+  // own-site index, opaque so that per-site LDS addresses are formed where
+  // they are used rather than held across the time loop
+  auto site = [&](int u) {
+    int p = threadIdx.x + u * kSmdThreads;
+    asm volatile("" : "+v"(p));
+    return p;
+  };
+  double qss_prev = 0.0;   // qSS partial of the previous step, reduced with this step's sums
+  int t_prev = -1;
   for (int it = 0; it < a.n_points; it++) {
     const int t = a.t0 + it;
     if (t > 0) {
-      T K1[SPT][3], K2[SPT][3], K3[SPT][3];
-      // stage 1: K1 from S, B = S + K1/2
+      T K1[SPT][3], A2[SPT][3];
+      double eref_part = 0.0;
+      // stage 1: K1 from S, B1 = S + K1/2 (and the energy of S from its fields)
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           T hx, hy, hz;
-          local_field<T, ZC>(L.cp, L.nb, Jt, p, Sx, Sy, Sz, hx, hy, hz);
+          field(u, p, Sx, Sy, Sz, hx, hy, hz);
           const T x = Sx[p], y = Sy[p], z = Sz[p];
+          if (fe) eref_part += 0.5 * ((double)x * hx + (double)y * hy + (double)z * hz) - 0.5 * hext * z;
           T px, py, pz;
           cross3(hx, hy, hz, x, y, z, px, py, pz);
           K1[u][0] = dt * px; K1[u][1] = dt * py; K1[u][2] = dt * pz;
-          Bx[p] = x + T(0.5) * K1[u][0];
-          By[p] = y + T(0.5) * K1[u][1];
-          Bz[p] = z + T(0.5) * K1[u][2];
+          Px[p] = x + T(0.5) * K1[u][0];
+          Py[p] = y + T(0.5) * K1[u][1];
+          Pz[p] = z + T(0.5) * K1[u][2];
         }
       }
       __syncthreads();
-      // stage 2
+      // stage 2: K2 from B1, B2 = S + K2/2
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           T hx, hy, hz;
-          local_field<T, ZC>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          field(u, p, Px, Py, Pz, hx, hy, hz);
           T px, py, pz;
-          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
-          K2[u][0] = dt * px; K2[u][1] = dt * py; K2[u][2] = dt * pz;
+          cross3(hx, hy, hz, Px[p], Py[p], Pz[p], px, py, pz);
+          A2[u][0] = dt * px; A2[u][1] = dt * py; A2[u][2] = dt * pz;
+          Qx[p] = Sx[p] + T(0.5) * A2[u][0];
+          Qy[p] = Sy[p] + T(0.5) * A2[u][1];
+          Qz[p] = Sz[p] + T(0.5) * A2[u][2];
         }
       }
       __syncthreads();
+      // stage 3: K3 from B2, B1 = S + K3, A2 = K2 + K3
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
-        if (p < N) {
-          Bx[p] = Sx[p] + T(0.5) * K2[u][0];
-          By[p] = Sy[p] + T(0.5) * K2[u][1];
-          Bz[p] = Sz[p] + T(0.5) * K2[u][2];
-        }
-      }
-      __syncthreads();
-      // stage 3
-#pragma unroll
-      for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           T hx, hy, hz;
-          local_field<T, ZC>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          field(u, p, Qx, Qy, Qz, hx, hy, hz);
           T px, py, pz;
-          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
-          K3[u][0] = dt * px; K3[u][1] = dt * py; K3[u][2] = dt * pz;
+          cross3(hx, hy, hz, Qx[p], Qy[p], Qz[p], px, py, pz);
+          const T k3[3] = {dt * px, dt * py, dt * pz};
+          Px[p] = Sx[p] + k3[0];
+          Py[p] = Sy[p] + k3[1];
+          Pz[p] = Sz[p] + k3[2];
+#pragma unroll
+          for (int d = 0; d < 3; d++) A2[u][d] += k3[d];
         }
       }
       __syncthreads();
+      // stage 4 and combine, S written in place
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
-        if (p < N) {
-          Bx[p] = Sx[p] + K3[u][0];
-          By[p] = Sy[p] + K3[u][1];
-          Bz[p] = Sz[p] + K3[u][2];
-        }
-      }
-      __syncthreads();
-      // stage 4 and combine (S is not read by any thread in this stage)
-      T Sn[SPT][3];
-#pragma unroll
-      for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           T hx, hy, hz;
-          local_field<T, ZC>(L.cp, L.nb, Jt, p, Bx, By, Bz, hx, hy, hz);
+          field(u, p, Px, Py, Pz, hx, hy, hz);
           T px, py, pz;
-          cross3(hx, hy, hz, Bx[p], By[p], Bz[p], px, py, pz);
+          cross3(hx, hy, hz, Px[p], Py[p], Pz[p], px, py, pz);
           const T f1 = T(1.0 / 6.0), f2 = T(2.0 / 6.0);
           T v[3];
           const T k4[3] = {dt * px, dt * py, dt * pz};
           const T s[3] = {Sx[p], Sy[p], Sz[p]};
 #pragma unroll
-          for (int d = 0; d < 3; d++)
-            v[d] = s[d] + (f1 * (K1[u][d] + k4[d]) + f2 * (K2[u][d] + K3[u][d]));
+          for (int d = 0; d < 3; d++) v[d] = s[d] + (f1 * (K1[u][d] + k4[d]) + f2 * A2[u][d]);
           const T inrm = T(1) / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-          Sn[u][0] = v[0] * inrm; Sn[u][1] = v[1] * inrm; Sn[u][2] = v[2] * inrm;
+          Sx[p] = v[0] * inrm; Sy[p] = v[1] * inrm; Sz[p] = v[2] * inrm;
         }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
-        if (p < N) { Sx[p] = Sn[u][0]; Sy[p] = Sn[u][1]; Sz[p] = Sn[u][2]; }
       }
       __syncthreads();
       // energy correction: sum |h x S|^2 and E(S) in FP64
       double D[SPT][3];
-      double v2[2] = {0.0, 0.0};
+      double v4[4] = {0.0, 0.0, eref_part, qss_prev};
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           T hx, hy, hz;
-          local_field<T, ZC>(L.cp, L.nb, Jt, p, Sx, Sy, Sz, hx, hy, hz);
+          field(u, p, Sx, Sy, Sz, hx, hy, hz);
           const double x = Sx[p], y = Sy[p], z = Sz[p];
+          if (fe) v4[1] += 0.5 * (x * hx + y * hy + z * hz) - 0.5 * hext * z;
           double px, py, pz;
           cross3<double>(hx, hy, hz, x, y, z, px, py, pz);
-          v2[0] += px * px + py * py + pz * pz;
+          v4[0] += px * px + py * py + pz * pz;
           cross3<double>(x, y, z, px, py, pz, D[u][0], D[u][1], D[u][2]);
         }
       }
-      v2[1] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
-      block_sum<2>(v2, scratch);
-      const double cfac = (Eref - v2[1]) / v2[0];
-      double v3[2] = {0.0, 0.0};
+      if (!fe) v4[1] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
+      block_sum<4>(v4, scratch);   // ends with a barrier: S may be overwritten below
+      if (fe) Eref = v4[2];
+      if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v4[3] / (double)N;
+      const double cfac = (Eref - v4[1]) / v4[0];
+      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * 3 * N : nullptr;
+      double qs = 0.0;
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
-        const int p = threadIdx.x + u * kSmdThreads;
+        const int p = site(u);
         if (p < N) {
           double x = (double)Sx[p] + cfac * D[u][0];
           double y = (double)Sy[p] + cfac * D[u][1];
           double z = (double)Sz[p] + cfac * D[u][2];
           const double nr = sqrt(x * x + y * y + z * z);
           x /= nr; y /= nr; z /= nr;
-          v3[1] += (double)g0[p] * x + (double)g0[Npad + p] * y + (double)g0[2 * Npad + p] * z;
-          Sx[p] = (T)x; Sy[p] = (T)y; Sz[p] = (T)z;
+          qs += spin0(u, p, 0) * x + spin0(u, p, 1) * y + spin0(u, p, 2) * z;
+          const T tx = (T)x, ty = (T)y, tz = (T)z;
+          Sx[p] = tx; Sy[p] = ty; Sz[p] = tz;
+          if (row) { row[p] = tx; row[N + p] = ty; row[2 * N + p] = tz; }
         }
       }
       __syncthreads();
-      v3[0] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
-      block_sum<2>(v3, scratch);
-      Eref = v3[0];
-      if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t] = v3[1] / (double)N;
+      qss_prev = qs;
+      t_prev = t;
+      if (!fe) {
+        double v1[1] = {bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz)};
+        block_sum<1>(v1, scratch);
+        Eref = v1[0];
+      }
     } else {
       if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + 0] = 1.0;
-    }
-    // record the configuration at time t
-    if (a.traj) {
-      T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
-      for (int p = threadIdx.x; p < N; p += blockDim.x) {
-        row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
+      if (a.traj) {
+        T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
+        for (int p = threadIdx.x; p < N; p += blockDim.x) {
+          row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
+        }
       }
     }
+  }
+  // flush the last step's qSS and, with field energies, the energy of the
+  // final configuration (the next launch's E_ref)
+  {
+    double e_part = 0.0;
+    if (fe) {
+#pragma unroll
+      for (int u = 0; u < SPT; u++) {
+        const int p = site(u);
+        if (p < N) {
+          T hx, hy, hz;
+          field(u, p, Sx, Sy, Sz, hx, hy, hz);
+          const double x = Sx[p], y = Sy[p], z = Sz[p];
+          e_part += 0.5 * (x * hx + y * hy + z * hz) - 0.5 * hext * z;
+        }
+      }
+    }
+    double v2[2] = {e_part, qss_prev};
+    block_sum<2>(v2, scratch);
+    if (fe) Eref = v2[0];
+    if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v2[1] / (double)N;
   }
   if (threadIdx.x == 0) a.eref[sidx] = Eref;
   if (LDS) {
@@ -282,37 +440,48 @@ struct SmdLaunch {
   int S;
   int spt;
   int maxZ;
+  bool general;   // some lattice needs per-slot J tables
   bool use_lds;
   size_t lds_bytes;
 };
 
-void smd_launch_f32(const SmdLaunch &c, StepArgs a);
-void smd_launch_f64(const SmdLaunch &c, StepArgs a);
-
-// Launches the time-step kernel for element type T: all samples in one
-// launch with LDS-resident lattices, or (global-memory form) one launch per
-// sample.
-template <typename T, int SPT, int ZC>
+// LDS-resident form: S, B1, B2 of one sample fit 150 KB only for N <= 4096
+// (FP32) or N <= 2048 (FP64), i.e. SPT x sizeof(T) <= 16.
+template <typename T, int SPT, int ZC, bool GEN>
 void smd_launch_t(const SmdLaunch &c, StepArgs a) {
   if (c.use_lds) {
-    const void *fn = (const void *)smd_kernel<T, SPT, true, ZC>;
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes));
-    a.sample_base = 0;
-    hipLaunchKernelGGL((smd_kernel<T, SPT, true, ZC>), dim3(c.S), dim3(kSmdThreads), c.lds_bytes, 0, a);
+    if constexpr (SPT * sizeof(T) <= 16) {
+      const void *fn = (const void *)smd_kernel<T, SPT, true, ZC, GEN>;
+      HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes));
+      a.sample_base = 0;
+      hipLaunchKernelGGL((smd_kernel<T, SPT, true, ZC, GEN>), dim3(c.S), dim3(kSmdThreads), c.lds_bytes,
+                         0, a);
+    } else {
+      throw Error(HSMD_ERR_ARG, "LDS-resident SMD form requested for a lattice that does not fit");
+    }
   } else {
     for (int s = 0; s < c.S; s++) {
       a.sample_base = s;
-      hipLaunchKernelGGL((smd_kernel<T, SPT, false, ZC>), dim3(1), dim3(kSmdThreads), 0, 0, a);
+      hipLaunchKernelGGL((smd_kernel<T, SPT, false, ZC, GEN>), dim3(1), dim3(kSmdThreads), 0, 0, a);
     }
   }
 }
 
+// Slot-count classes: 4 (square, kagome, hexagonal and Lieb J1), 8, 12, 20
+// (triangular and kagome up to the third shell) and 32; lattices with per-slot
+// J tables take the 32-slot form with run-time slot tests.
 template <typename T, int SPT>
 void smd_launch_zc(const SmdLaunch &c, const StepArgs &a) {
   const int Z = c.maxZ;
-  if (Z <= 8) smd_launch_t<T, SPT, 8>(c, a);
-  else if (Z <= 16) smd_launch_t<T, SPT, 16>(c, a);
-  else smd_launch_t<T, SPT, 32>(c, a);
+  if (c.general) smd_launch_t<T, SPT, 32, true>(c, a);
+  else if (Z <= 4) smd_launch_t<T, SPT, 4, false>(c, a);
+  else if (Z <= 8) smd_launch_t<T, SPT, 8, false>(c, a);
+  else if (Z <= 12) smd_launch_t<T, SPT, 12, false>(c, a);
+  else if (Z <= 20) smd_launch_t<T, SPT, 20, false>(c, a);
+  else smd_launch_t<T, SPT, 32, false>(c, a);
 }
+
+void smd_launch_f32(const SmdLaunch &c, StepArgs a);   // smd_f32.hip
+void smd_launch_f64(const SmdLaunch &c, StepArgs a);   // smd_f64.hip
 
 }  // namespace hsmd

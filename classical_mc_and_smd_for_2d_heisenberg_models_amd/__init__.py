@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libhsmd.so")
 
 PLAIN_F64, BATCHED_F32, BATCHED_F64 = 0, 1, 2
 HEISENBERG, ISING = 0, 1
-N_MOMENTS = 19
+N_MOMENTS = 66
 MOMENT_NAMES = ["E", "E2", "F1", "F2", "F4", "T1", "T2", "Q1", "Q2", "Q4",
                 "S1", "S2", "S4", "SH", "SV", "SD", "X1", "X2", "X4"]
 
@@ -95,6 +95,9 @@ SIGNATURES = {
     "hsmd_mc_reset_counters": (_I, [_P]),
     "hsmd_mc_moments": (_I, [_P, _D, _I64, _I]),
     "hsmd_mc_slot_map": (_I, [_P, _I32]),
+    "hsmd_mc_set_sq": (_I, [_P, _I]),
+    "hsmd_mc_sq_field": (_I, [_P, _D, _I64, _I]),
+    "hsmd_static_sq": (_I, [_P, _I, _D, _D]),
     "hsmd_mc_destroy": (None, [_P]),
     "hsmd_smd_create": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D,
                              ctypes.POINTER(_P)]),
@@ -209,6 +212,12 @@ class Lattice:
         _check(lib().hsmd_lattice_tables(self._h, None, _dp(rv)))
         return rv
 
+    def positions(self):
+        """Site coordinates rvecList (N x 2)."""
+        rv = np.zeros((self.N, 2))
+        _check(lib().hsmd_lattice_tables(self._h, None, _dp(rv)))
+        return rv
+
     def grid_map(self):
         g = np.zeros(self.Nq, np.int32)
         _check(lib().hsmd_lattice_grid_map(self._h, _ip(g)))
@@ -238,6 +247,14 @@ class Lattice:
         out = np.zeros(s.shape[0])
         _check(lib().hsmd_energies(self._h, precision, s.shape[0], _dp(s), _dp(out)))
         return out
+
+    def static_sq(self, spins):
+        """get_SqData_iMKL / get_qct_SqData of R configurations: complex
+        (R, 3, Nq) in the reference's q order."""
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, self.N, 3)
+        out = np.zeros((s.shape[0], 3, self.Nq, 2))
+        _check(lib().hsmd_static_sq(self._h, s.shape[0], _dp(s), _dp(out)))
+        return out[..., 0] + 1j * out[..., 1]
 
     # reference-named aliases
     get_localField = local_fields
@@ -304,6 +321,18 @@ class MonteCarlo:
         m = np.zeros(self.R, np.int32)
         _check(lib().hsmd_mc_slot_map(self._h, _ip(m)))
         return m
+
+    def set_sq(self, every):
+        """Static S(q) measurement every `every` measured sweeps (0 = off)."""
+        _check(lib().hsmd_mc_set_sq(self._h, int(every)))
+
+    def sq_field(self, reset=False):
+        """(R, Nq, 6) sums of |S_x|, |S_y|, |S_z|, arg S_x, arg S_y, arg S_z and
+        the number of S(q) measurements."""
+        out = np.zeros((self.R, self.lattice.Nq, 6))
+        n = np.zeros(1, np.int64)
+        _check(lib().hsmd_mc_sq_field(self._h, _dp(out), n.ctypes.data_as(_I64), int(reset)))
+        return out, int(n[0])
 
 
 class SpinDynamics:

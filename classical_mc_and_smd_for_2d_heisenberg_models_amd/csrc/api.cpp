@@ -26,6 +26,9 @@ void mc_pt_counters(MCEngine &e, int64_t *att, int64_t *acc);
 void mc_reset_counters(MCEngine &e);
 void mc_moments(MCEngine &e, double *out, int64_t *n, int reset);
 void mc_slot_map(MCEngine &e, int32_t *m);
+void mc_set_sq(MCEngine &e, int every);
+void mc_sq_field(MCEngine &e, double *out, int64_t *count, int reset);
+void static_sq(const Lattice &L, int n_configs, const double *spins, double *out);
 void mc_destroy(MCEngine *e);
 void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
                      double *energies);
@@ -258,6 +261,28 @@ hsmd_status hsmd_mc_moments(hsmd_mc *mc, double *out, int64_t *n_meas, int reset
   return guarded([&] {
     REQUIRE(mc && out, "null argument");
     mc_moments(*mc->e, out, n_meas, reset);
+  });
+}
+
+hsmd_status hsmd_mc_set_sq(hsmd_mc *mc, int every) {
+  return guarded([&] {
+    REQUIRE(mc, "null argument");
+    mc_set_sq(*mc->e, every);
+  });
+}
+
+hsmd_status hsmd_mc_sq_field(hsmd_mc *mc, double *out, int64_t *count, int reset) {
+  return guarded([&] {
+    REQUIRE(mc, "null argument");
+    mc_sq_field(*mc->e, out, count, reset);
+  });
+}
+
+hsmd_status hsmd_static_sq(const hsmd_lattice *lat, int n_configs, const double *spins,
+                           double *out) {
+  return guarded([&] {
+    REQUIRE(lat && spins && out, "null argument");
+    static_sq(lat->lat, n_configs, spins, out);
   });
 }
 

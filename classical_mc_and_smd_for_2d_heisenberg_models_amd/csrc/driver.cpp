@@ -386,13 +386,15 @@ std::vector<double> temperature_grid(const Settings &s, int npt) {
 
 std::string t_tag(double T, const std::string &label) { return "_T(" + fixed(T, 5) + ")" + label; }
 
-// Per-bin averages of CF/mc_hm_code.cpp:5376-5636 (Values1; spin stiffness
-// and the S(q)-based Values2 are not computed here and are written as 0).
-void bin_values(const double *mom, int64_t n, int N, double beta, bool ising, bool c3,
-                bool z2, double v1[23]) {
+// Per-bin averages of CF/mc_hm_code.cpp:5376-5636: Values1 (energy,
+// Binder cumulants, susceptibilities, order parameters, spin stiffness) and
+// Values2 (S(q) peaks and correlation lengths at the five key points, scaled
+// by 1/(measCnt2 Gsz)).  Entries with |v| <= 1e-12 (or NaN) are written as 0.
+void bin_values(const double *mom, int64_t n, int N, int Gsz, double beta, bool ising, bool c3,
+                double v1[23], double v2[40]) {
   const double fc1 = 1.0 / (double)n;
-  double a[19];
-  for (int k = 0; k < 19; k++) a[k] = fc1 * mom[k];
+  double a[25];
+  for (int k = 0; k < 25; k++) a[k] = fc1 * mom[k];
   const double E1 = a[0], E2 = a[1], F1 = a[2], F2 = a[3], F4 = a[4], T1 = a[5], T2 = a[6];
   const double Q1 = a[7], Q2 = a[8], Q4 = a[9], S1 = a[10], S2 = a[11], S4 = a[12];
   const double SH = a[13], SV = a[14], SD = a[15], X1 = a[16], X2 = a[17], X4 = a[18];
@@ -411,13 +413,38 @@ void bin_values(const double *mom, int64_t n, int N, double beta, bool ising, bo
   }
   if (c3) XB = 1.5 * (X2 == 0 ? 0 : 1.0 - X4 / (2.0 * X2 * X2));
   else XB = 1.5 * (X2 == 0 ? 0 : 1.0 - X4 / (3.0 * X2 * X2));
-  (void)z2;
+  // spin stiffness, fac_RhoSt = -2 / (sqrt(3) N) (CF/mc_hm_code.cpp:3741, 5411-5415)
+  const double fr = -2.0 / (sqrt(3.0) * N);
+  const double RhoS1 = fr * (a[19] + a[20] + a[21]) / 3.0;
+  const double RhoS2 = beta * fr * (a[22] + a[23] + a[24]) / 3.0;
   const double vals[23] = {EDen1, EDen2, fh * (EDen2 - EDen1 * EDen1),
                            QB, fs * (Q2 - Q1 * Q1), SB, fs * (S2 - S1 * S1),
                            XB, fs * (X2 - X1 * X1), FB, fs * (F2 - F1 * F1),
                            fs * (T2 - T1 * T1), Q1, S1, fabs(X1), F1, T1,
-                           SH, SV, SD, 0.0, 0.0, 0.0};
+                           SH, SV, SD, RhoS1, RhoS2, RhoS1 + RhoS2};
   for (int k = 0; k < 23; k++) v1[k] = fabs(vals[k]) > 1.0e-12 ? vals[k] : 0.0;
+  const double fc2 = 1.0 / mom[65];
+  for (int k = 0; k < 40; k++) {
+    const double v = (fc2 / Gsz) * mom[25 + k];
+    v2[k] = fabs(v) > 1.0e-12 ? v : 0.0;
+  }
+}
+
+// MC_Bin_w0SqField (CF/mc_hm_code.cpp:5752-5800): int32 count, then per q
+// the complex (sum |S_a|) e^{i <arg S_a>} for a = x, y, z.
+void write_w0_sq_field(const std::string &path, const double *f, int Nq, int64_t count) {
+  std::ofstream out(path, std::ios::binary | std::ios::trunc);
+  if (!out) fail(HSMD_ERR_IO, "cannot write " + path);
+  const int32_t cnt = (int32_t)count;
+  out.write(reinterpret_cast<const char *>(&cnt), sizeof(cnt));
+  const double fc = 1.0 / (double)count;
+  for (int q = 0; q < Nq; q++) {
+    for (int d = 0; d < 3; d++) {
+      const double amp = f[(size_t)q * 6 + d], ph = fc * f[(size_t)q * 6 + 3 + d];
+      const double z[2] = {amp * cos(ph), amp * sin(ph)};
+      out.write(reinterpret_cast<const char *>(z), sizeof(z));
+    }
+  }
 }
 
 }  // namespace
@@ -460,6 +487,11 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
   if (s.NTerm > n0) check(hsmd_mc_run(mc, s.NTerm - n0, pt ? 2 : -1, 0, nullptr));
   // measurement: NBins bins of BinSz sweeps, pair parity = bin index % 2
   check(hsmd_mc_reset_counters(mc));
+  // S(q) every dftDelay1 sweeps of a bin: 10, or BinSz for tabulated graphs
+  // (CF/subrouts.cpp:2143-2153)
+  const bool dft = s.code[WITH_DFT] != 0;
+  const int Gsz = info[6], Nq = info[7];
+  if (dft) check(hsmd_mc_set_sq(mc, s.code[QCRYSTAL] ? kBinSz : 10));
   std::vector<double> dummy(HSMD_N_MOMENTS * (size_t)npt);
   int64_t nm = 0;
   check(hsmd_mc_moments(mc, dummy.data(), &nm, 1));
@@ -489,14 +521,22 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
     }
     check(hsmd_mc_moments(mc, mom.data(), &nm, 1));
     for (int k = 0; k < npt; k++) {
-      double v1[23];
-      bin_values(&mom[(size_t)k * HSMD_N_MOMENTS], nm, N, 1.0 / T[k], s.code[ISI_MODEL] != 0, c3,
-                 s.geom == "square", v1);
+      double v1[23], v2[40];
+      bin_values(&mom[(size_t)k * HSMD_N_MOMENTS], nm, N, Gsz, 1.0 / T[k], s.code[ISI_MODEL] != 0,
+                 c3, v1, v2);
       const double zeros[45] = {0};
       write_bin(f1[k], v1, 23, true);
-      write_bin(f2[k], zeros, 40, true);
+      write_bin(f2[k], v2, 40, true);
       write_bin(f3[k], zeros, 45, true);
     }
+  }
+  if (dft) {
+    std::vector<double> sqf((size_t)npt * Nq * 6);
+    int64_t cnt = 0;
+    check(hsmd_mc_sq_field(mc, sqf.data(), &cnt, 0));
+    for (int k = 0; k < npt; k++)
+      write_w0_sq_field(dir + "/outBin/MC_Bin_w0SqField" + t_tag(T[k], s.label) + ".bin",
+                        &sqf[(size_t)k * Nq * 6], Nq, cnt);
   }
   write_bin(dir + "/outBin/Temp_List" + s.label + ".bin", T.data(), T.size(), false);
 }

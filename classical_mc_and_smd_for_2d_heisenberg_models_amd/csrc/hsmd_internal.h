@@ -43,6 +43,7 @@ struct Lattice {
   int N = 0, Gsz = 0, Nq = 0;
   int zn1 = 0, zn2 = 0, znx = 0;
   bool tabulated = false;    // qcrystal-style input
+  bool pbc = true;           // periodic boundaries (generated lattices)
   std::vector<int32_t> nbors;    // N x (zn1+zn2+znx), -1 = missing
   std::vector<double> rvec;      // N x 2
   std::vector<int32_t> gridMap;  // Gsz*Gsz -> site or -1 (periodic lattices)
@@ -80,6 +81,10 @@ struct Lattice {
   // field couplings equal to bond couplings); checked on a test configuration.
   bool field_energy_ok() const;
   double bondJ(int shell, int i, int j) const;   // J of a bond (energy)
+  double stiffJ(int shell, int i, int j) const;  // J of a bond in the spin stiffness
+  // static S(q): direct sums (tabulated or open lattices) or the grid DFT
+  bool sq_direct() const { return tabulated || !pbc; }
+  double sq_dq() const;   // momentum step dq (CF/mc_hm_code.cpp:1486)
   double slotJ(int shell, int k, int i, int nb) const;  // J of a field slot
   double lam(int shell) const { return shell == 1 ? (xxz_on ? c.lambda1 : 1.0)
                                                   : (xxz_on ? c.lambda2 : 1.0); }

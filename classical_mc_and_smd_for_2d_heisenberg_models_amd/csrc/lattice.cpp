@@ -67,6 +67,21 @@ double Lattice::bondJ(int shell, int i, int j) const {
   return J;
 }
 
+// get_spinStiff (CF/subrouts.cpp:6594-6604): the shell coupling, or the
+// disorder matrix entry, without the Lieb J2 asymmetry.
+double Lattice::stiffJ(int shell, int i, int j) const {
+  const int a1 = imp.empty() ? 0 : imp[i], a2 = imp.empty() ? 0 : imp[j];
+  if (shell == 1) return disorder ? (a1 == a2 ? (a1 ? c.J1ii : c.J1) : c.J1ic) : c.J1;
+  if (shell == 2) return disorder ? (a1 == a2 ? (a1 ? c.J2ii : c.J2) : c.J2ic) : c.J2;
+  return disorder ? (a1 == a2 ? (a1 ? c.JXii : c.JX) : c.JXic) : c.JX;
+}
+
+// dq = 4 (2 pi) / Gsz for tabulated graphs, 2 pi / Gsz otherwise.
+double Lattice::sq_dq() const {
+  const double pi2 = 2.0 * acos(-1.0);
+  return tabulated ? 4.0 * pi2 / Gsz : pi2 / Gsz;
+}
+
 double Lattice::slotJ(int shell, int k, int i, int nb) const {
   double J;
   const int a1 = imp.empty() ? 0 : imp[i], a2 = imp.empty() ? 0 : imp[nb];
@@ -372,6 +387,7 @@ Lattice *lattice_generate(const std::string &geom, int L, bool pbc, bool strict,
     make_sites(*lat, gi);
     make_grid_map(*lat, gi);
     make_nbors(*lat, gi, pbc);
+    lat->pbc = pbc;
     lat->imp.assign(lat->N, 0);
     lat->vacant.assign(lat->N, 0);
     make_labels(*lat);

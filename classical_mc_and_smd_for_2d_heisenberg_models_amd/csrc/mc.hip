@@ -28,10 +28,13 @@
 
 #include "device_common.h"
 #include "hsmd_internal.h"
+#include "sq.h"
 
 namespace hsmd {
 
-constexpr int NOBS = 32;   // raw order-parameter sums per configuration
+constexpr int NOBS = 32;   // scratch width of the sweep kernels' reductions
+constexpr int NOBSX = 40;  // per-configuration sums of obs_kernel: order
+                           // parameters [0..31], stiffness C-sums [32..37]
 constexpr int NMOM = HSMD_N_MOMENTS;
 constexpr int kThreads = 1024;
 
@@ -60,7 +63,8 @@ struct SweepArgs {
   const int32_t *slot_of_config;
   const int32_t *boost_slot;
   double *energy;          // per configuration
-  double *obs;             // per configuration, NOBS
+  double *obs;             // per configuration, NOBSX
+  const double *stiff_w;   // 6 x Nb: J C0^2, J C1^2, J C2^2, J C0, J C1, J C2
 };
 
 // ------------------------------------------------------------ site updates
@@ -224,16 +228,20 @@ __device__ __forceinline__ void load_spin<int8_t>(const int8_t *g, int Npad, int
   x = (double)g[p]; y = 0.0; z = 0.0;
 }
 
+// Order-parameter sums and the spin-stiffness bond sums of get_spinStiff
+// (CF/subrouts.cpp:6519-6620): dprod = J (x_i x_j + y_i y_j),
+// vprod = J (x_i y_j - y_i x_j), weighted by C^2 and C for the three lattice
+// directions (weights precomputed per bond).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
-  __shared__ double scratch[NOBS * 16];
+  __shared__ double scratch[NOBSX * 16];
   const int c = blockIdx.x;
   const int N = a.cp.N, Npad = a.cp.Npad;
   const int comps = sizeof(T) == 1 ? 1 : 3;
   const T *g = (const T *)a.spins + (size_t)c * comps * Npad;
-  double v[NOBS];
+  double v[NOBSX];
 #pragma unroll
-  for (int k = 0; k < NOBS; k++) v[k] = 0.0;
+  for (int k = 0; k < NOBSX; k++) v[k] = 0.0;
   for (int p = threadIdx.x; p < N; p += blockDim.x) {
     double x, y, z;
     load_spin<T>(g, Npad, p, x, y, z);
@@ -252,10 +260,22 @@ __global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
       }
     }
   }
-  block_sum<NOBS>(v, scratch);
+  const int Nb = a.bd.Nb;
+  for (int b = threadIdx.x; b < Nb; b += blockDim.x) {
+    double xi, yi, zi, xj, yj, zj;
+    load_spin<T>(g, Npad, a.bi[b], xi, yi, zi);
+    load_spin<T>(g, Npad, a.bj[b], xj, yj, zj);
+    const double dp = xi * xj + yi * yj, vp = xi * yj - yi * xj;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[32 + k] += a.stiff_w[(size_t)k * Nb + b] * dp;
+      v[35 + k] += a.stiff_w[(size_t)(3 + k) * Nb + b] * vp;
+    }
+  }
+  block_sum<NOBSX>(v, scratch);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NOBS; k++) a.obs[(size_t)c * NOBS + k] = v[k];
+    for (int k = 0; k < NOBSX; k++) a.obs[(size_t)c * NOBSX + k] = v[k];
   }
 }
 
@@ -516,7 +536,7 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
     const double E = a.energy[c];
     if (a.etrace) a.etrace[k] = E;
     if (a.measure) {
-      const double *o = a.obs + (size_t)c * NOBS;
+      const double *o = a.obs + (size_t)c * NOBSX;
       double *m = a.moments + (size_t)k * NMOM;
       const double F1 = iN * sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
       const double F2 = F1 * F1;
@@ -547,6 +567,9 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
       m[10] += S1; m[11] += S2; m[12] += S2 * S2;
       m[13] += SH; m[14] += SV; m[15] += SD;
       m[16] += (a.z2 && !a.c3) ? fabs(X1) : X1; m[17] += X2; m[18] += X2 * X2;
+      // spin stiffness: rhoS1vec summed, rhoS2vec = squared C-sums
+      m[19] += o[32]; m[20] += o[33]; m[21] += o[34];
+      m[22] += o[35] * o[35]; m[23] += o[36] * o[36]; m[24] += o[37] * o[37];
       if (k == 0) a.n_meas[0] += 1;
     }
   }
@@ -607,6 +630,13 @@ struct MCEngine {
   DevBuf<double> beta, energy, obs, moments, etrace;
   DevBuf<int32_t> cfg_of_slot, slot_of_cfg, boost;
   DevBuf<long long> pt_cnt, pt_exc, n_meas;
+  DevBuf<double> stiff_w;
+  // static S(q) measurement
+  int sq_every = 0;
+  long long meas_count = 0;   // measured sweeps since the last moments reset
+  std::unique_ptr<SqPlan> sq;
+  DevBuf<double> sqfield;
+  long long sq_count = 0;
   SweepArgs args{};
 };
 
@@ -712,7 +742,27 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   for (int k = 0; k < R; k++) b[k] = 1.0 / temps[k];
   e->beta.upload(b.data(), R);
   e->energy.alloc(R); e->energy.zero();
-  e->obs.alloc((size_t)R * NOBS); e->obs.zero();
+  e->obs.alloc((size_t)R * NOBSX); e->obs.zero();
+  {
+    // stiffness weights per bond (new labels, reference bond order)
+    const double c60 = 0.5, s60 = sqrt(3.0) / 2.0;
+    const size_t Nb = lat.bi.size();
+    std::vector<double> w(6 * std::max<size_t>(Nb, 1), 0.0);
+    if (lat.rvec.size() == (size_t)2 * N) {
+      for (size_t b = 0; b < Nb; b++) {
+        const int i = lat.bi[b], j = lat.bj[b];
+        const double dx = lat.rvec[2 * i] - lat.rvec[2 * j];
+        const double dy = lat.rvec[2 * i + 1] - lat.rvec[2 * j + 1];
+        const double C[3] = {dx, s60 * dy + c60 * dx, s60 * dy - c60 * dx};
+        const double J = lat.stiffJ(lat.bshell[b], i, j);
+        for (int k = 0; k < 3; k++) {
+          w[k * Nb + b] = J * C[k] * C[k];
+          w[(3 + k) * Nb + b] = J * C[k];
+        }
+      }
+    }
+    e->stiff_w.upload(w.data(), w.size());
+  }
   e->moments.alloc((size_t)R * NMOM); e->moments.zero();
   std::vector<int32_t> id(R);
   for (int k = 0; k < R; k++) id[k] = k;
@@ -747,6 +797,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   a.boost_slot = e->boost.p;
   a.energy = e->energy.p;
   a.obs = e->obs.p;
+  a.stiff_w = e->stiff_w.p;
   {
     int maxcol = 0;
     for (int k = 0; k < lat.n_colours; k++) maxcol = std::max(maxcol, lat.col_off[k + 1] - lat.col_off[k]);
@@ -913,6 +964,13 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
         hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
       else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
     }
+    if (measure && e.sq_every > 0 && e.meas_count % e.sq_every == 0) {
+      const SpinType ty = e.ising ? SPIN_I8 : (e.prec == HSMD_BATCHED_F32 ? SPIN_F32 : SPIN_F64);
+      sq_compute(*e.sq, e.spins.p, ty, e.R);
+      sq_accumulate(*e.sq, e.R, e.cfg_of_slot.p, e.sqfield.p, e.moments.p, NMOM, 25);
+      e.sq_count++;
+    }
+    if (measure) e.meas_count++;
     if (post) {
       p.sweep = e.sweep;
       p.parity = pt_parity == 2 ? (int)(e.sweep % 2u) : pt_parity;
@@ -975,7 +1033,30 @@ void mc_moments(MCEngine &e, double *out, int64_t *n, int reset) {
   long long nm = 0;
   HIP_CHECK(hipMemcpy(&nm, e.n_meas.p, sizeof(long long), hipMemcpyDeviceToHost));
   if (n) *n = nm;
-  if (reset) { e.moments.zero(); e.n_meas.zero(); }
+  if (reset) { e.moments.zero(); e.n_meas.zero(); e.meas_count = 0; }
+}
+
+void mc_set_sq(MCEngine &e, int every) {
+  if (every < 0) throw Error(HSMD_ERR_ARG, "S(q) interval must be >= 0");
+  if (every > 0 && !e.sq) {
+    std::unique_ptr<SqPlan> p(new SqPlan());
+    sq_plan_init(*p, e.lat, e.R);
+    e.sqfield.alloc((size_t)e.R * p->Nq * 6);
+    e.sqfield.zero();
+    e.sq = std::move(p);
+    e.sq_count = 0;
+  }
+  e.sq_every = every;
+}
+
+void mc_sq_field(MCEngine &e, double *out, int64_t *count, int reset) {
+  if (!e.sq) throw Error(HSMD_ERR_STATE, "S(q) measurement was never enabled");
+  HIP_CHECK(hipDeviceSynchronize());
+  if (out)
+    HIP_CHECK(hipMemcpy(out, e.sqfield.p, (size_t)e.R * e.sq->Nq * 6 * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  if (count) *count = e.sq_count;
+  if (reset) { e.sqfield.zero(); e.sq_count = 0; }
 }
 
 void mc_slot_map(MCEngine &e, int32_t *m) {

@@ -160,10 +160,37 @@ hsmd_status hsmd_mc_pt_counters(hsmd_mc *mc, int64_t *attempts, int64_t *accepts
 hsmd_status hsmd_mc_reset_counters(hsmd_mc *mc);
 /* Observable moments accumulated per slot since the last reset:
  * out[R x HSMD_N_MOMENTS], n_meas = number of measured sweeps.
- * Moment order: E, E^2, F1, F2, F4, T1, T2, Q1, Q2, Q4, S1, S2, S4, SH, SV, SD,
- *               X1, X2, X4 (X = C3 Psi or Z2 Ising parameter). */
-#define HSMD_N_MOMENTS 19
+ * Moment order (CF/mc_hm_code.cpp:4951-5245):
+ *   [0..18]  E, E^2, F1, F2, F4, T1, T2, Q1, Q2, Q4, S1, S2, S4, SH, SV, SD,
+ *            X1, X2, X4 (X = C3 Psi or Z2 Ising parameter);
+ *   [19..24] spin stiffness (get_spinStiff, CF/subrouts.cpp:6519-6620):
+ *            sum rhoS1vec[0..2], sum rhoS2vec[0..2] (the squared C-sums);
+ *   [25..44] S(q) peak values SQ1 at the five key points, xy, yz, zz, tt
+ *            (get_corrLen's first output, stored by the reference as
+ *            xyCLen..ttCLen, Values2[0..19]);
+ *   [45..64] correlation lengths (get_corrLen's second output, xySVal..ttSVal,
+ *            Values2[20..39]);
+ *   [65]     number of S(q) measurements (measCnt2).
+ * [25..65] accumulate only when S(q) measurement is on (hsmd_mc_set_sq). */
+#define HSMD_N_MOMENTS 66
 hsmd_status hsmd_mc_moments(hsmd_mc *mc, double *out, int64_t *n_meas, int reset);
+/* Static S(q) measurement during measured sweeps: at every `every`-th
+ * measured sweep counted from the last hsmd_mc_moments reset, starting with
+ * the first (the reference's nbin % dftDelay1 == 0 within a bin,
+ * CF/mc_hm_code.cpp:5185-5245); 0 turns it off.
+ * Periodic generated lattices use the grid DFT of get_SqData_iMKL
+ * (CF/subrouts.cpp:8330-8443), tabulated or open ones the direct sums of
+ * get_qct_SqData (CF/subrouts.cpp:8460-8525). */
+hsmd_status hsmd_mc_set_sq(hsmd_mc *mc, int every);
+/* Accumulated S(q) per slot, out[R x Nq x 6] = sum |S_x|, |S_y|, |S_z|,
+ * arg S_x, arg S_y, arg S_z (the reference's SqField,
+ * CF/mc_hm_code.cpp:5205-5213); count = number of S(q) measurements. */
+hsmd_status hsmd_mc_sq_field(hsmd_mc *mc, double *out, int64_t *count, int reset);
+/* Static S(q) of n_configs configurations (n_configs x n x 3 f64, the
+ * reference's site order): out[n_configs x 3 x Nq x 2] (re, im), the output
+ * of get_SqData_iMKL / get_qct_SqData (same choice as hsmd_mc_set_sq). */
+hsmd_status hsmd_static_sq(const hsmd_lattice *lat, int n_configs, const double *spins,
+                           double *out);
 /* Slot -> configuration map (R int32). */
 hsmd_status hsmd_mc_slot_map(hsmd_mc *mc, int32_t *config_of_slot);
 void hsmd_mc_destroy(hsmd_mc *mc);

@@ -920,4 +920,137 @@ int orc_window(const char *name, int N, double *w) {
   return 1;
 }
 
+// --- spin stiffness (get_spinStiff, CF/subrouts.cpp:6519-6620) ------------
+// Per configuration: rhoS1vec (sum J C^2 (x_i x_j + y_i y_j)) for the three
+// directions C0 = dx, C1 = c60 dx + s60 dy, C2 = s60 dy - c60 dx, with raw
+// position differences, then the C-sums of J (x_i y_j - y_i x_j) (before the
+// reference squares them).  J: shell coupling or disorder matrix entry.
+void orc_spin_stiffness(void *mp, const double *rvec, int R, const double *spins, double *out) {
+  const Model &m = *(const Model *)mp;
+  const double c60 = cos(PI / 3.0), s60 = sin(PI / 3.0);
+  for (int r = 0; r < R; r++) {
+    const double *S = spins + (size_t)r * m.N * 3;
+    double a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+    for (size_t b = 0; b < m.bi.size(); b++) {
+      const int i = m.bi[b], j = m.bj[b], sh = m.bs[b];
+      const double dx = rvec[2 * i] - rvec[2 * j], dy = rvec[2 * i + 1] - rvec[2 * j + 1];
+      const double C[3] = {dx, c60 * dx + s60 * dy, s60 * dy - c60 * dx};
+      double J;
+      if (m.dis) J = jmat(m, sh, m.imp[i], m.imp[j]);
+      else J = sh == 1 ? m.J1 : (sh == 2 ? m.J2 : m.JX);
+      const double dp = J * (S[3 * i] * S[3 * j] + S[3 * i + 1] * S[3 * j + 1]);
+      const double vp = J * (S[3 * i] * S[3 * j + 1] - S[3 * i + 1] * S[3 * j]);
+      for (int k = 0; k < 3; k++) {
+        a1[k] += C[k] * C[k] * dp;
+        a2[k] += C[k] * vp;
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      out[(size_t)r * 6 + k] = a1[k];
+      out[(size_t)r * 6 + 3 + k] = a2[k];
+    }
+  }
+}
+
+// --- static S(q) on the DFT grid (get_SqData_iMKL, CF/subrouts.cpp:8330-8443)
+// Textbook 2D DFT of the gridMap-scattered spins (MKL forward, unscaled),
+// times 1/sqrt(Nq); out[R][3][G*G][2], index i * G + j.
+void orc_sq_grid(int N, int G, const int32_t *gridmap, int R, const double *spins, double *out) {
+  const int Nq = G * G;
+  const double fc = 1.0 / sqrt((double)Nq);
+  std::vector<double> cs(G), sn(G);
+  for (int m = 0; m < G; m++) { cs[m] = cos(2.0 * PI * m / G); sn[m] = sin(2.0 * PI * m / G); }
+  for (int r = 0; r < R; r++)
+    for (int d = 0; d < 3; d++)
+      for (int qi = 0; qi < G; qi++)
+        for (int qj = 0; qj < G; qj++) {
+          double re = 0.0, im = 0.0;
+          for (int i = 0; i < G; i++)
+            for (int j = 0; j < G; j++) {
+              const int k = gridmap[i * G + j];
+              if (k < 0) continue;
+              const double v = spins[((size_t)r * N + k) * 3 + d];
+              const int mm = (int)(((long long)i * qi + (long long)j * qj) % G);
+              re += v * cs[mm];
+              im -= v * sn[mm];
+            }
+          double *o = out + (((size_t)r * 3 + d) * Nq + (size_t)qi * G + qj) * 2;
+          o[0] = fc * re;
+          o[1] = fc * im;
+        }
+}
+
+// --- static S(q) by direct sums (get_qct_SqData, CF/subrouts.cpp:8460-8525)
+// q = dq (nx, ny), nx = i or i - L (i >= L/2), likewise ny from j; output
+// index j * L + i of an Nq-long array (entries beyond L^2 stay 0).
+void orc_sq_direct(int N, int L, int Nq, double dq, const double *rvec, int R,
+                   const double *spins, double *out) {
+  const int l2 = L / 2;
+  const double fc = 1.0 / sqrt((double)N);
+  memset(out, 0, sizeof(double) * (size_t)R * 3 * Nq * 2);
+  for (int r = 0; r < R; r++)
+    for (int j = 0; j < L; j++) {
+      const int ny = j < l2 ? j : j - L;
+      for (int i = 0; i < L; i++) {
+        const int nx = i < l2 ? i : i - L;
+        std::complex<double> S[3];
+        for (int k = 0; k < N; k++) {
+          const std::complex<double> f =
+              std::exp(std::complex<double>(0.0, -dq) * (nx * rvec[2 * k] + ny * rvec[2 * k + 1]));
+          for (int d = 0; d < 3; d++) S[d] += spins[((size_t)r * N + k) * 3 + d] * f;
+        }
+        for (int d = 0; d < 3; d++) {
+          double *o = out + (((size_t)r * 3 + d) * Nq + (size_t)j * L + i) * 2;
+          o[0] = fc * S[d].real();
+          o[1] = fc * S[d].imag();
+        }
+      }
+    }
+}
+
+// --- key points of the ordered spectrum (get_StaticSFac CF/subrouts.cpp:8799,
+// get_OrderedSpecArray2D :8868, get_corrLen :9335).  sq: [3][G*G][2] of one
+// configuration.  peaks[4][5] (xy, yz, zz, tt) = SQ1 at the five points,
+// clen[4][5] = sqrt(SQ1/SQ2 - 1)/dq or 0.
+void orc_corr_len(int G, int c3, double dq, const double *sq, double *peaks, double *clen) {
+  const int Nq = G * G, l2 = G / 2, l3 = G / 3, l4 = G / 4;
+  std::vector<int> mv(G);
+  for (int i = 0; i < G; i++) { const int n = 2 * (i / l2); mv[i] = i - l2 * (n - 1); }
+  // ordered arrays xy, yz, zz, tt
+  std::vector<double> ord((size_t)4 * Nq);
+  for (int i = 0; i < G; i++)
+    for (int j = 0; j < G; j++) {
+      const int k = mv[i] * G + mv[j];
+      double p[3];
+      for (int d = 0; d < 3; d++) {
+        const double re = sq[((size_t)d * Nq + k) * 2], im = sq[((size_t)d * Nq + k) * 2 + 1];
+        p[d] = std::max(1.0e-16, re * re + im * im);
+      }
+      ord[(size_t)0 * Nq + i * G + j] = 0.5 * (p[0] + p[1]);
+      ord[(size_t)1 * Nq + i * G + j] = 0.5 * (p[1] + p[2]);
+      ord[(size_t)2 * Nq + i * G + j] = p[2];
+      ord[(size_t)3 * Nq + i * G + j] = (1.0 / 3.0) * (p[0] + p[1] + p[2]);
+    }
+  int i1[5], j1[5], i2[5], j2[5];
+  if (c3) {
+    i1[0] = l3 + l2; j1[0] = -l3 + l2; i2[0] = l3 + l2; j2[0] = -l3 + l2 + 1;
+    i1[1] = l2; j1[1] = 0; i2[1] = l2; j2[1] = 1;
+    i1[2] = 0; j1[2] = l2; i2[2] = 1; j2[2] = l2;
+  } else {
+    i1[0] = 0; j1[0] = 0; i2[0] = 0; j2[0] = 1;
+    i1[1] = 0; j1[1] = l2; i2[1] = 1; j2[1] = l2;
+    i1[2] = l2; j1[2] = 0; i2[2] = l2; j2[2] = 1;
+  }
+  i1[3] = l2; j1[3] = l2; i2[3] = l2; j2[3] = l2 + 1;
+  i1[4] = l4; j1[4] = l4; i2[4] = l4; j2[4] = l4 + 1;
+  for (int f = 0; f < 4; f++)
+    for (int k = 0; k < 5; k++) {
+      const double a = ord[(size_t)f * Nq + i1[k] * G + j1[k]];
+      const double b = ord[(size_t)f * Nq + i2[k] * G + j2[k]];
+      const double w = a / b;
+      peaks[f * 5 + k] = a;
+      clen[f * 5 + k] = w > 1.0 ? (1.0 / dq) * sqrt(w - 1.0) : 0.0;
+    }
+}
+
 }  // extern "C"

@@ -41,6 +41,10 @@ _SIG = {
     "orc_smd_run": (None, [_P, _D, ctypes.c_double, _I, _I, _D, _D, _D]),
     "orc_dssf": (None, [_I, _I, _I32, _I, _D, _D, _I, _D]),
     "orc_window": (_I, [ctypes.c_char_p, _I, _D]),
+    "orc_spin_stiffness": (None, [_P, _D, _I, _D, _D]),
+    "orc_sq_grid": (None, [_I, _I, _I32, _I, _D, _D]),
+    "orc_sq_direct": (None, [_I, _I, _I, ctypes.c_double, _D, _I, _D, _D]),
+    "orc_corr_len": (None, [_I, _I, ctypes.c_double, _D, _D, _D]),
     "orc_observables": (None, [ctypes.c_char_p, _I, _I, _I32, _I, _I, _I, _I, _D, _D]),
 }
 
@@ -168,6 +172,14 @@ class Model:
                                _dp(acc), threads)
         return cos, acc
 
+    def spin_stiffness(self, rvec, spins):
+        """(R, 6): rhoS1vec[0..2] and the unsquared C-sums of rhoS2vec."""
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, self.N, 3)
+        rv = np.ascontiguousarray(rvec, dtype=np.float64)
+        out = np.zeros((s.shape[0], 6))
+        lib().orc_spin_stiffness(self._h, _dp(rv), s.shape[0], _dp(s), _dp(out))
+        return out
+
     def smd_run(self, spins, dt, n_points, gauss_seidel=True, traj=True):
         s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.N, 3).copy()
         tr = np.zeros((n_points, self.N, 3)) if traj else None
@@ -186,6 +198,36 @@ def dssf(traj, gridmap, G, window, n_omega):
     out = np.zeros((3, G * G, n_omega))
     lib().orc_dssf(N, G, _ip(gm), T, _dp(traj), _dp(w), n_omega, _dp(out))
     return out
+
+
+def sq_grid(gridmap, G, spins):
+    """Static S(q) on the DFT grid: complex (R, 3, G*G)."""
+    s = np.ascontiguousarray(spins, dtype=np.float64)
+    s = s.reshape(-1, s.shape[-2], 3)
+    gm = np.ascontiguousarray(gridmap, dtype=np.int32)
+    out = np.zeros((s.shape[0], 3, G * G, 2))
+    lib().orc_sq_grid(s.shape[1], G, _ip(gm), s.shape[0], _dp(s), _dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def sq_direct(rvec, L, Nq, dq, spins):
+    """Static S(q) by direct sums: complex (R, 3, Nq)."""
+    s = np.ascontiguousarray(spins, dtype=np.float64)
+    s = s.reshape(-1, s.shape[-2], 3)
+    rv = np.ascontiguousarray(rvec, dtype=np.float64)
+    out = np.zeros((s.shape[0], 3, Nq, 2))
+    lib().orc_sq_direct(s.shape[1], L, Nq, dq, _dp(rv), s.shape[0], _dp(s), _dp(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def corr_len(sq, G, c3, dq):
+    """Key points of one configuration's S(q) (3, G*G complex):
+    (peaks[4][5], clen[4][5]) for xy, yz, zz, tt."""
+    z = np.ascontiguousarray(np.stack([np.real(sq), np.imag(sq)], -1), dtype=np.float64)
+    pk = np.zeros((4, 5))
+    cl = np.zeros((4, 5))
+    lib().orc_corr_len(G, int(c3), dq, _dp(z), _dp(pk), _dp(cl))
+    return pk, cl
 
 
 def window(name, n):

@@ -98,6 +98,7 @@ SIGNATURES = {
     "hsmd_mc_set_sq": (_I, [_P, _I]),
     "hsmd_mc_sq_field": (_I, [_P, _D, _I64, _I]),
     "hsmd_static_sq": (_I, [_P, _I, _D, _D]),
+    "hsmd_smd_dssf_q": (_I, [_P, _I, _I, _D, _D]),
     "hsmd_mc_destroy": (None, [_P]),
     "hsmd_smd_create": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D,
                              ctypes.POINTER(_P)]),
@@ -383,6 +384,14 @@ class SpinDynamics:
         q = np.zeros(self.n_steps)
         _check(lib().hsmd_smd_dssf(self._h, li, _dp(out), _dp(q)))
         return out, q
+
+    def dssf_q(self, q, li=0):
+        """Sample-summed DSSF of lattice li at wavevectors q (nq x 2) by direct
+        spatial sums: (3, nq, n_omega)."""
+        qv = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((3, qv.shape[0], self.n_omega))
+        _check(lib().hsmd_smd_dssf_q(self._h, li, qv.shape[0], _dp(qv), _dp(out)))
+        return out
 
     def energies(self):
         e = np.zeros(self.S)

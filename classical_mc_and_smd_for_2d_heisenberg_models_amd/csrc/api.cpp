@@ -41,6 +41,7 @@ void smd_set_samples(SMDEngine &e, const double *spins);
 void smd_get_samples(SMDEngine &e, double *spins);
 void smd_step(SMDEngine &e, int n);
 void smd_dssf(SMDEngine &e, int li, double *out, double *qss);
+void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out);
 void smd_energies(SMDEngine &e, double *E);
 void smd_destroy(SMDEngine *e);
 
@@ -334,6 +335,13 @@ hsmd_status hsmd_smd_step(hsmd_smd *smd, int n_steps) {
   return guarded([&] {
     REQUIRE(smd, "null argument");
     smd_step(*smd->e, n_steps);
+  });
+}
+
+hsmd_status hsmd_smd_dssf_q(hsmd_smd *smd, int li, int nq, const double *q, double *out) {
+  return guarded([&] {
+    REQUIRE(smd && q && out, "null argument");
+    smd_dssf_q(*smd->e, li, nq, q, out);
   });
 }
 

@@ -322,8 +322,8 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
         g[k] = (site < 0 || L.vacant[site]) ? -1 : L.inv[site];
       }
       e->gmap[i].upload(g.data(), g.size());
-    } else if (sp.dssf) {
-      throw Error(HSMD_ERR_ARG, "DSSF on tabulated graphs needs the direct-sum transform (not built)");
+    } else if (sp.dssf && L.rvec.size() != (size_t)2 * L.N) {
+      throw Error(HSMD_ERR_ARG, "DSSF on a tabulated graph needs its site positions");
     }
   }
   e->dlats.upload(hl.data(), hl.size());
@@ -497,6 +497,74 @@ void smd_step(SMDEngine &e, int n) {
 }
 
 template <typename T>
+void dssf_direct_launch(const SampleDev *samples, const int *ids, int n_samples, int N, int nq,
+                        int nop, const double *q, const double *pos, const uint8_t *mask, T *P,
+                        const T *Y, double scale, double *dssf);
+
+// DSSF at wavevectors q (nq x 2) by direct spatial sums over the sites of
+// lattice li (vacant sites excluded): out[3][nq][n_omega], sample-summed
+// |X|^2 / (N T).
+template <typename T>
+static void dssf_direct(SMDEngine &e, int li, int nq, const double *q, double *out) {
+  const Lattice &L = e.lats[li];
+  if (L.rvec.size() != (size_t)2 * L.N) throw Error(HSMD_ERR_ARG, "lattice has no site positions");
+  const int N = L.N, no = e.nop, nkeep = e.spec.n_omega;
+  std::vector<int> ids;
+  for (int s = 0; s < e.S; s++) if (e.sample_lat[s] == li) ids.push_back(s);
+  std::vector<double> pos((size_t)2 * N);
+  std::vector<uint8_t> mask(N);
+  for (int p = 0; p < N; p++) {
+    const int i = L.perm[p];
+    pos[2 * p] = L.rvec[2 * (size_t)i];
+    pos[2 * p + 1] = L.rvec[2 * (size_t)i + 1];
+    mask[p] = L.vacant.empty() ? 0 : (uint8_t)(L.vacant[i] != 0);
+  }
+  DevBuf<double> dpos, dq, d;
+  DevBuf<uint8_t> dmask;
+  DevBuf<int> dids;
+  dpos.upload(pos.data(), pos.size());
+  dmask.upload(mask.data(), mask.size());
+  if (!ids.empty()) dids.upload(ids.data(), ids.size());
+  // the phase table is built per chunk of wavevectors, at most 2^28 entries
+  const int qc = std::max(1, std::min(nq, (int)((1ll << 28) / std::max(1, N))));
+  DevBuf<T> P;
+  P.alloc((size_t)qc * N * 2);
+  d.alloc((size_t)3 * qc * no);
+  const double scale = 1.0 / ((double)N * (double)e.spec.n_steps);
+  std::vector<double> h((size_t)3 * qc * no);
+  for (int q0 = 0; q0 < nq; q0 += qc) {
+    const int nc = std::min(qc, nq - q0);
+    d.zero();
+    if (!ids.empty()) {
+      dq.upload(q + 2 * (size_t)q0, (size_t)2 * nc);
+      dssf_direct_launch<T>(e.dsamples.p, dids.p, (int)ids.size(), N, nc, no, dq.p, dpos.p,
+                            dmask.p, P.p, (const T *)e.ybuf.p, scale, d.p);
+    }
+    HIP_CHECK(hipMemcpy(h.data(), d.p, (size_t)3 * nc * no * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; c++)
+      for (int k = 0; k < nc; k++)
+        for (int n = 0; n < nkeep; n++)
+          out[((size_t)c * nq + q0 + k) * nkeep + n] = h[((size_t)c * nc + k) * no + n];
+  }
+}
+
+// The reference's wavevector grid of tabulated graphs: q = dq (nx, ny),
+// nx, ny in [0, L/2) then [-L/2, 0), at index ny_index * L + nx_index
+// (get_qct_SqData, CF/subrouts.cpp:8460-8525); dq = 4 (2 pi) / Gsz.
+static std::vector<double> tabulated_q_grid(const Lattice &L) {
+  const int Lq = L.L, l2 = Lq / 2;
+  const double dq = L.sq_dq();
+  std::vector<double> q((size_t)2 * L.Nq, 0.0);
+  for (int j = 0; j < Lq; j++)
+    for (int i = 0; i < Lq; i++) {
+      const int nx = i < l2 ? i : i - Lq, ny = j < l2 ? j : j - Lq;
+      q[2 * ((size_t)j * Lq + i)] = dq * nx;
+      q[2 * ((size_t)j * Lq + i) + 1] = dq * ny;
+    }
+  return q;
+}
+
+template <typename T>
 static void dssf_space(SMDEngine &e, int li, double *out) {
   const Lattice &L = e.lats[li];
   const int G = L.Gsz, no = e.nop, nkeep = e.spec.n_omega, Nq = G * G;
@@ -536,7 +604,15 @@ void smd_dssf(SMDEngine &e, int li, double *out, double *qss) {
   if (!e.spec.dssf) throw Error(HSMD_ERR_STATE, "DSSF accumulation is off");
   if (e.t_next != e.spec.n_steps) throw Error(HSMD_ERR_STATE, "DSSF needs all n_steps time points");
   flush_chunk(e);
-  if (e.f32) dssf_space<float>(e, li, out); else dssf_space<double>(e, li, out);
+  const Lattice &L = e.lats[li];
+  if (L.tabulated) {
+    // get_dynSpectrum takes get_qct_SqData for tabulated graphs (CF/subrouts.cpp:11906-11908)
+    const std::vector<double> q = tabulated_q_grid(L);
+    if (e.f32) dssf_direct<float>(e, li, L.Nq, q.data(), out);
+    else dssf_direct<double>(e, li, L.Nq, q.data(), out);
+  } else {
+    if (e.f32) dssf_space<float>(e, li, out); else dssf_space<double>(e, li, out);
+  }
   if (qss) {
     const int T_ = e.spec.n_steps;
     std::vector<double> q((size_t)e.S * T_);
@@ -546,6 +622,16 @@ void smd_dssf(SMDEngine &e, int li, double *out, double *qss) {
       if (e.sample_lat[s] == li)
         for (int t = 0; t < T_; t++) qss[t] += q[(size_t)s * T_ + t];
   }
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out) {
+  if (li < 0 || li >= (int)e.lats.size()) throw Error(HSMD_ERR_ARG, "bad lattice index");
+  if (nq < 1) throw Error(HSMD_ERR_ARG, "nq must be >= 1");
+  if (!e.spec.dssf) throw Error(HSMD_ERR_STATE, "DSSF accumulation is off");
+  if (e.t_next != e.spec.n_steps) throw Error(HSMD_ERR_STATE, "DSSF needs all n_steps time points");
+  flush_chunk(e);
+  if (e.f32) dssf_direct<float>(e, li, nq, q, out); else dssf_direct<double>(e, li, nq, q, out);
   HIP_CHECK(hipDeviceSynchronize());
 }
 

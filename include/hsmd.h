@@ -223,10 +223,19 @@ hsmd_status hsmd_smd_get_samples(hsmd_smd *smd, double *spins);
 hsmd_status hsmd_smd_step(hsmd_smd *smd, int n_steps);
 /* Sample-summed DSSF of lattice li: out[3][Nq][n_omega] f64 =
  * sum_s |X_s(q, w)|^2 with X the windowed, 1/sqrt(Nq T)-normalised space-time
- * DFT (CF/subrouts.cpp:11906-11925, 12062-12105), and qss[n_steps] the
+ * DFT (CF/subrouts.cpp:11906-11925, 12062-12105); for tabulated graphs the
+ * spatial transform is get_qct_SqData's direct sum on its L x L q grid,
+ * normalised 1/sqrt(N T) (CF/subrouts.cpp:8460-8525), and qss[n_steps] the
  * sample-summed autocorrelation (1/N) sum_i S_i(0).S_i(t).  Available after
  * n_steps - 1 steps. Divide by the sample count for the reference's averages. */
 hsmd_status hsmd_smd_dssf(hsmd_smd *smd, int li, double *out, double *qss);
+/* Sample-summed DSSF of lattice li at nq arbitrary wavevectors q[nq x 2]
+ * (a q-path), by direct spatial sums: out[3][nq][n_omega] f64 =
+ * sum_s |sum_r e^{-i q.r} Y_s(r, w)|^2 / (N T), Y the windowed time DFT of the
+ * spins; the transform get_qct_SqData applies to tabulated graphs
+ * (CF/subrouts.cpp:8460-8525), at the caller's q.  Vacant sites are left out.
+ * No counterpart call in the reference, which fixes q to its grid. */
+hsmd_status hsmd_smd_dssf_q(hsmd_smd *smd, int li, int nq, const double *q, double *out);
 /* Energies of every sample (S f64). */
 hsmd_status hsmd_smd_energies(hsmd_smd *smd, double *energies);
 void hsmd_smd_destroy(hsmd_smd *smd);

@@ -769,6 +769,36 @@ void orc_dssf(int N, int G, const int32_t *gridmap, int T, const double *traj,
     }
 }
 
+// --- DSSF by direct spatial sums (get_dynSpectrum with get_qct_SqData,
+// CF/subrouts.cpp:11906-11912, 8460-8525, 12062-12105): S(q, t) =
+// (1/sqrt(N)) sum_r e^{-i q.r} S(r, t) at the given wavevectors (sites with
+// mask[r] != 0 left out), then the windowed time DFT, |X|^2 / T.
+void orc_dssf_direct(int N, const double *rvec, const int32_t *mask, int nq, const double *qv,
+                     int T, const double *traj, const double *window, int n_omega, double *out) {
+  typedef std::complex<double> C;
+  const double fc = sqrt(1.0 / N), sfc = sqrt(1.0 / T);
+  std::vector<C> hist((size_t)3 * T), ph(N), twt(T);
+  for (int k = 0; k < T; k++) twt[k] = std::polar(1.0, -2.0 * PI * k / T);
+  for (int q = 0; q < nq; q++) {
+    for (int r = 0; r < N; r++)
+      ph[r] = (mask && mask[r]) ? C(0, 0)
+                                : std::exp(C(0.0, -1.0) * (qv[2 * q] * rvec[2 * r] + qv[2 * q + 1] * rvec[2 * r + 1]));
+    for (int t = 0; t < T; t++)
+      for (int d = 0; d < 3; d++) {
+        C acc(0, 0);
+        for (int r = 0; r < N; r++) acc += traj[((size_t)t * N + r) * 3 + d] * ph[r];
+        hist[(size_t)d * T + t] = fc * acc;
+      }
+    for (int d = 0; d < 3; d++)
+      for (int n = 0; n < n_omega; n++) {
+        C acc(0, 0);
+        for (int t = 0; t < T; t++) acc += window[t] * hist[(size_t)d * T + t] * twt[((long long)n * t) % T];
+        const C x = sfc * acc;
+        out[((size_t)d * nq + q) * n_omega + n] = std::real(x * std::conj(x));
+      }
+  }
+}
+
 
 // --- order parameters (CF/subrouts.cpp:5362-5800) ---------------------------
 // geom: "square", "triang", "hexagn", "kagome", "lieb"; spins N x 3.

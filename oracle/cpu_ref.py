@@ -45,6 +45,7 @@ _SIG = {
     "orc_sq_grid": (None, [_I, _I, _I32, _I, _D, _D]),
     "orc_sq_direct": (None, [_I, _I, _I, ctypes.c_double, _D, _I, _D, _D]),
     "orc_corr_len": (None, [_I, _I, ctypes.c_double, _D, _D, _D]),
+    "orc_dssf_direct": (None, [_I, _D, _I32, _I, _D, _I, _D, _D, _I, _D]),
     "orc_observables": (None, [ctypes.c_char_p, _I, _I, _I32, _I, _I, _I, _I, _D, _D]),
 }
 
@@ -228,6 +229,20 @@ def corr_len(sq, G, c3, dq):
     cl = np.zeros((4, 5))
     lib().orc_corr_len(G, int(c3), dq, _dp(z), _dp(pk), _dp(cl))
     return pk, cl
+
+
+def dssf_direct(traj, rvec, q, window, n_omega, mask=None):
+    """DSSF by direct sums at wavevectors q (nq x 2): (3, nq, n_omega)."""
+    traj = np.ascontiguousarray(traj, dtype=np.float64)
+    T, N = traj.shape[0], traj.shape[1]
+    rv = np.ascontiguousarray(rvec, dtype=np.float64)
+    qv = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 2)
+    w = np.ascontiguousarray(window, dtype=np.float64)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+    out = np.zeros((3, qv.shape[0], n_omega))
+    lib().orc_dssf_direct(N, _dp(rv), None if mk is None else _ip(mk), qv.shape[0], _dp(qv), T,
+                          _dp(traj), _dp(w), n_omega, _dp(out))
+    return out
 
 
 def window(name, n):

@@ -62,7 +62,19 @@ def test_mc_then_smd_pipeline_files_and_dssf(tmp_path):
         v1 = v1.reshape(2, 23)
         # energy density below the paramagnet's and bounded by the ground state
         assert np.all(v1[:, 0] < 0.1) and np.all(v1[:, 0] > -1.25)
-        assert np.fromfile(os.path.join(d, "outBin", "MC_Bin_AvgValue2" + tag)).size == 80
+        v2 = np.fromfile(os.path.join(d, "outBin", "MC_Bin_AvgValue2" + tag))
+        assert v2.size == 80
+        v2 = v2.reshape(2, 40)
+        # S(q) peaks (xy, yz, zz, tt at five points) are positive, lengths >= 0
+        assert np.all(v2[:, :20] > 0) and np.all(v2[:, 20:] >= 0)
+        # spin stiffness RhoS1 (a bond sum over every sweep) is measured
+        assert np.all(v1[:, 20] != 0)
+        # MC_Bin_w0SqField: int32 count (every 10th sweep of 2 bins) + Nq x 3 complex
+        raw = open(os.path.join(d, "outBin", "MC_Bin_w0SqField" + tag), "rb").read()
+        assert len(raw) == 4 + N * 3 * 16
+        assert np.frombuffer(raw[:4], np.int32)[0] == 200
+        w0 = np.frombuffer(raw[4:], np.complex128).reshape(N, 3)
+        assert np.all(np.abs(w0) > 0)
         sf = np.fromfile(os.path.join(d, "outBin", "MC_Bin_SpinField" + tag)).reshape(-1, N, 3)
         assert sf.shape[0] == 2000 // 500
         np.testing.assert_allclose(np.linalg.norm(sf, axis=-1), 1.0, atol=1e-12)

@@ -1,4 +1,5 @@
 """GPU parity tests of the SMD engine (RK4 + energy correction + DSSF)."""
+import math
 import sys
 
 import numpy as np
@@ -119,3 +120,79 @@ def test_dssf_fp32_within_1e3_of_fp64(no):
     err = np.abs(a - b).max() / a.max()
     _say("fp32 dssf max err / peak", err)
     assert err < 1e-3
+
+
+def _qc_fixture(n):
+    import os
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qc_with_PBC_%d" % n)
+    nb = np.fromfile(os.path.join(d, "nborsList_AppxtSz(%d).bin" % n), dtype="<i4").reshape(n, 19)
+    rv = np.fromfile(os.path.join(d, "sitesList_AppxtSz(%d).bin" % n), dtype="<f8").reshape(n, 2)
+    return nb, rv
+
+
+@pytest.mark.parametrize("prec,tol", [(H.BATCHED_F64, 1e-8), (H.BATCHED_F32, 2e-4)])
+def test_dssf_direct_sums_on_tabulated_graph_match_oracle(prec, tol):
+    # tabulated graphs take get_qct_SqData's direct sums on the L x L q grid
+    _say("dssf direct tabulated", prec)
+    nb, rv = _qc_fixture(41)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
+    lat = H.Lattice.from_tables(nb, 8, 3, 8, rv, c)
+    m = cpu_ref.Model(nb, (8, 3, 8), c.as_params())
+    rng = np.random.default_rng(12)
+    S, T, no = 2, 64, 32
+    samples = [_spins(rng, lat.N) for _ in range(S)]
+    w = cpu_ref.window("hann1", T)
+    smd = H.SpinDynamics([lat], [0] * S, 0.02, T, n_omega=no, precision=prec, window=w)
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    d, _ = smd.dssf(0)
+    Lq = lat.L
+    dq = 4 * 2 * math.pi / lat.Gsz
+    qv = np.zeros((lat.Nq, 2))
+    for j in range(Lq):
+        for i in range(Lq):
+            nx = i if i < Lq // 2 else i - Lq
+            ny = j if j < Lq // 2 else j - Lq
+            qv[j * Lq + i] = (dq * nx, dq * ny)
+    ref = np.zeros_like(d)
+    for s0 in samples:
+        _, tr, _, _ = m.smd_run(s0, 0.02, T, gauss_seidel=False)
+        ref += cpu_ref.dssf_direct(tr, rv, qv, w, no)
+    np.testing.assert_allclose(d, ref, rtol=tol, atol=tol * ref.max())
+    # a q-path through the same lattice
+    path = np.stack([np.linspace(0, 3.0, 17), np.linspace(0, -1.5, 17)], 1)
+    dp = smd.dssf_q(path)
+    refp = np.zeros_like(dp)
+    for s0 in samples:
+        _, tr, _, _ = m.smd_run(s0, 0.02, T, gauss_seidel=False)
+        refp += cpu_ref.dssf_direct(tr, rv, path, w, no)
+    np.testing.assert_allclose(dp, refp, rtol=tol, atol=tol * refp.max())
+
+
+def test_dssf_q_on_a_periodic_lattice_with_vacancies_matches_oracle():
+    _say("dssf_q square with vacancies")
+    c = H.Couplings(J1=1.0, J2=0.5)
+    lat = H.Lattice.generate("square", 6, c)
+    rng = np.random.default_rng(13)
+    vac = (rng.random(lat.N) < 0.15).astype(np.int32)
+    lat.set_vacancies(vac)
+    S, T, no = 2, 48, 32
+    samples = [_spins(rng, lat.N) for _ in range(S)]
+    w = np.ones(T)
+    smd = H.SpinDynamics([lat], [0] * S, 0.02, T, n_omega=no, precision=H.BATCHED_F64)
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    path = np.stack([np.linspace(0, math.pi, 9), np.linspace(0, math.pi, 9)], 1)
+    got = smd.dssf_q(path)
+    rv = lat.positions()
+    ref = np.zeros_like(got)
+    # vacancies: every table entry touching a vacant site is a hole
+    nb = lat.tables().copy()
+    for v in np.nonzero(vac)[0]:
+        nb[v, :] = -1
+        nb[nb == v] = -1
+    m = cpu_ref.Model(nb, (lat.zn1, lat.zn2, lat.znx), c.as_params())
+    for k in range(S):
+        _, tr, _, _ = m.smd_run(samples[k], 0.02, T, gauss_seidel=False)
+        ref += cpu_ref.dssf_direct(tr, rv, path, w, no, mask=vac)
+    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-10 * ref.max())

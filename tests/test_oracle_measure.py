@@ -139,3 +139,27 @@ def test_corr_len_k_point_on_triangular_120_state():
     assert pk[0, 0] == pytest.approx(N / 4, rel=1e-12)
     assert pk[3, 0] == pytest.approx(N / 6, rel=1e-12)
     assert pk[0, 3] < 1e-10            # Gamma empty
+
+
+def test_dssf_direct_equals_grid_dssf_on_square_lattice():
+    # square sites at integer positions: direct sums at q = 2 pi (a, b) / L
+    # reproduce the grid DSSF at FFT index (a, b) (N = Nq), with a window
+    L, T, no = 4, 16, 16
+    nb, rv, gm, zn = cpu_ref.lattice("square", L)
+    N = L * L
+    rng = np.random.default_rng(8)
+    traj = _random_spins(rng, T, N)
+    w = cpu_ref.window("hann1", T)
+    ref = cpu_ref.dssf(traj, gm, L, w, no)
+    # grid point k = a * L + b of the oracle grid DSSF: the site at grid (a', b')
+    # has position (a', b'), so q = 2 pi / L (a, b)
+    q = np.array([[2 * math.pi * (k // L) / L, 2 * math.pi * (k % L) / L] for k in range(N)])
+    got = cpu_ref.dssf_direct(traj, rv, q, w, no)
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+    # a masked site is removed from the sums
+    mask = np.zeros(N, np.int32)
+    mask[3] = 1
+    t2 = traj.copy()
+    t2[:, 3] = 0.0
+    np.testing.assert_allclose(cpu_ref.dssf_direct(traj, rv, q, w, no, mask=mask),
+                               cpu_ref.dssf_direct(t2, rv, q, w, no), atol=1e-12)

@@ -98,6 +98,8 @@ SIGNATURES = {
     "hsmd_mc_set_sq": (_I, [_P, _I]),
     "hsmd_mc_sq_field": (_I, [_P, _D, _I64, _I]),
     "hsmd_static_sq": (_I, [_P, _I, _D, _D]),
+    "hsmd_poisson_disk": (_I, [ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _I, _D, _I32]),
+    "hsmd_lattice_from_points": (_I, [_I, _D, ctypes.c_double, _D, _P, _P]),
     "hsmd_smd_dssf_q": (_I, [_P, _I, _I, _D, _D]),
     "hsmd_mc_destroy": (None, [_P]),
     "hsmd_smd_create": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D,
@@ -182,6 +184,17 @@ class Lattice:
         h = _P()
         _check(lib().hsmd_lattice_from_tables(n, zn1, zn2, znx, _ip(nbors),
                                                None if rv is None else _dp(rv),
+                                               ctypes.byref(c), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_points(cls, points, box, r_shell=(1.15, 1.45, 1.75), couplings=None):
+        """Tabulated lattice of points in a periodic box, shells by distance."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+        r = np.ascontiguousarray(r_shell, dtype=np.float64)
+        c = couplings or Couplings()
+        h = _P()
+        _check(lib().hsmd_lattice_from_points(pts.shape[0], _dp(pts), float(box), _dp(r),
                                                ctypes.byref(c), ctypes.byref(h)))
         return cls(h)
 
@@ -411,6 +424,15 @@ def get_dynSpectrum(lattice, samples, dt, n_steps, window=None, n_omega=None,
     smd.step(n_steps - 1)
     d, q = smd.dssf(0)
     return d / S, q / S
+
+
+def poisson_disk(box, r_min=1.0, seed=5, max_points=None):
+    """Seeded Poisson-disk points (n x 2) in the periodic box [0, box)^2."""
+    cap = int(max_points or (box / r_min) ** 2 * 1.2 + 16)
+    out = np.zeros((cap, 2))
+    n = np.zeros(1, np.int32)
+    _check(lib().hsmd_poisson_disk(float(box), float(r_min), int(seed), cap, _dp(out), _ip(n)))
+    return out[:int(n[0])].copy()
 
 
 def run_mc(workdir, n_replicas, precision=BATCHED_F32, seed=1, strict=True):

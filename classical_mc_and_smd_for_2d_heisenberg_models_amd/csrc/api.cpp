@@ -29,6 +29,9 @@ void mc_slot_map(MCEngine &e, int32_t *m);
 void mc_set_sq(MCEngine &e, int every);
 void mc_sq_field(MCEngine &e, double *out, int64_t *count, int reset);
 void static_sq(const Lattice &L, int n_configs, const double *spins, double *out);
+int poisson_disk(double box, double rmin, uint64_t seed, int max_n, double *out);
+Lattice *lattice_from_points(int n, const double *rvec, double box, const double r[3],
+                             const hsmd_couplings &c);
 void mc_destroy(MCEngine *e);
 void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
                      double *energies);
@@ -135,6 +138,28 @@ hsmd_status hsmd_lattice_set_vacancies(hsmd_lattice *lat, const int32_t *vac) {
     for (int i = 0; i < lat->lat.N; i++) REQUIRE(vac[i] == 0 || vac[i] == 1, "vacancy map must be 0/1");
     lat->lat.vacant.assign(vac, vac + lat->lat.N);
     lat->lat.rebuild();
+  });
+}
+
+hsmd_status hsmd_poisson_disk(double box, double r_min, uint64_t seed, int32_t max_points,
+                              double *points, int32_t *n_points) {
+  return guarded([&] {
+    REQUIRE(points && n_points, "null argument");
+    *n_points = poisson_disk(box, r_min, seed, max_points, points);
+  });
+}
+
+hsmd_status hsmd_lattice_from_points(int32_t n, const double *rvec, double box,
+                                     const double r_shell[3], const hsmd_couplings *c,
+                                     hsmd_lattice **out) {
+  return guarded([&] {
+    REQUIRE(rvec && r_shell && c && out, "null argument");
+    *out = nullptr;
+    Lattice *L = lattice_from_points(n, rvec, box, r_shell, *c);
+    hsmd_lattice *h = new hsmd_lattice();
+    h->lat = std::move(*L);
+    delete L;
+    *out = h;
   });
 }
 

@@ -85,11 +85,27 @@ hsmd_status hsmd_lattice_generate(const char *geometry, int L, int pbc, int stri
  * then zn2 shell-2, then znx shell-X int32 entries, -1 = missing (the
  * nborsList*.bin layout written at CF/subrouts.cpp:3294-3316, read at
  * CF/mc_hm_code.cpp:1273-1320).  rvec (n x 2 f64, may be NULL) is the
- * sitesList*.bin layout (CF/mc_hm_code.cpp:955-1020).  grid_L > 0 sets the
- * Lsz x Lsz direct-sum q-grid of get_qct_SqData (CF/subrouts.cpp:8460). */
+ * sitesList*.bin layout (CF/mc_hm_code.cpp:955-1020).  The direct-sum q
+ * grid of get_qct_SqData (CF/subrouts.cpp:8460) is Lsz x Lsz with Lsz the even
+ * integer part of sqrt(n) (CF/subrouts.cpp:1104-1112). */
 hsmd_status hsmd_lattice_from_tables(int32_t n, int zn1, int zn2, int znx,
                                      const int32_t *nbors, const double *rvec,
                                      const hsmd_couplings *c, hsmd_lattice **out);
+/* Seeded Poisson-disk point set in a periodic box [0, box)^2 with minimum
+ * distance r_min (Bridson's algorithm); writes up to max_points points
+ * (x, y f64) and their number.  Generates the irregular-graph inputs that the
+ * reference takes as tabulated files (its QCrystal_Data converters); no
+ * counterpart function in the reference. */
+hsmd_status hsmd_poisson_disk(double box, double r_min, uint64_t seed, int32_t max_points,
+                              double *points, int32_t *n_points);
+/* Tabulated lattice from points in a periodic box: shell 1 = minimum-image
+ * distance < r_shell[0], shell 2 < r_shell[1], shell X < r_shell[2]; slots by
+ * distance; widths = largest counts (an extended nborsList layout beyond the
+ * reference's fixed 8/3/8, CF/subrouts.cpp:1012).  Then as
+ * hsmd_lattice_from_tables. */
+hsmd_status hsmd_lattice_from_points(int32_t n, const double *rvec, double box,
+                                     const double r_shell[3], const hsmd_couplings *c,
+                                     hsmd_lattice **out);
 
 /* Sets the impurity map (n int32 of 0/1; CF/subrouts.cpp:3321-3352 draws it). */
 hsmd_status hsmd_lattice_set_impurities(hsmd_lattice *lat, const int32_t *imp);

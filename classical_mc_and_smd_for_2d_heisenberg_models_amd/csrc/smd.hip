@@ -171,6 +171,7 @@ struct SMDEngine {
   int S = 0;
   bool f32 = false;
   bool use_lds = false;
+  bool wide = false;
   int spt = 1;
   int maxZ = 0;
   bool general = false;
@@ -193,7 +194,8 @@ struct SMDEngine {
   DevBuf<LatDev> dlats;
   DevBuf<SampleDev> dsamples;
   DevBuf<unsigned char> spins, s0, stage, traj, ybuf, twid, dftcs;
-  DevBuf<double> qss, eref, dssf;
+  DevBuf<double> qss, eref, dssf, dd;
+  DevBuf<unsigned char> ks;
   DevBuf<int> sample_ids;
   DevBuf<GemmSampleHost> gemm_samples;
   DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 nop][ldt]
@@ -364,15 +366,19 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // launch shape
   e->spt = 1;
   while (e->spt * kSmdThreads < maxN) e->spt *= 2;
-  if (e->spt > 8 || (!e->f32 && e->spt > 4))
-    throw Error(HSMD_ERR_ARG, "lattice too large for the SMD kernel (N > 8192 FP32, N > 4096 FP64)");
   e->lds_npad = maxN + 1;
   e->lds_bytes = (size_t)3 * 3 * e->lds_npad * es;   // S, B1, B2
   e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
                e->spt * (int)es <= 16;
+  // register-resident global form only for the plain form's small lattices
+  e->wide = !e->use_lds && (sp.precision != HSMD_PLAIN_F64 || e->spt > (e->f32 ? 8 : 4));
   if (!e->use_lds) {
     e->stage.alloc(2 * so * es);
     e->stage.zero();
+  }
+  if (e->wide) {
+    e->ks.alloc(6 * so * es);
+    e->dd.alloc(3 * so);
   }
   return e.release();
 }
@@ -485,7 +491,10 @@ void smd_step(SMDEngine &e, int n) {
     a.lds_npad = e.lds_npad;
     a.stage = e.stage.p;
     a.stage_stride = (long long)(e.stage.n / (2 * (e.f32 ? sizeof(float) : sizeof(double))));
-    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.general, e.use_lds, e.lds_bytes};
+    a.ks = e.ks.p;
+    a.dd = e.dd.p;
+    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.general, e.use_lds, e.wide,
+                          e.spec.precision == HSMD_PLAIN_F64, e.lds_bytes};
     if (e.f32) smd_launch_f32(lc, a); else smd_launch_f64(lc, a);
     HIP_CHECK(hipGetLastError());
     e.t_next += np;

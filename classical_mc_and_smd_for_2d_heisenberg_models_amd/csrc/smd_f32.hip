@@ -6,6 +6,10 @@
 namespace hsmd {
 
 void smd_launch_f32(const SmdLaunch &c, StepArgs a) {
+  if (c.wide) {
+    smd_launch_wide<float>(c, a);
+    return;
+  }
   switch (c.spt) {
     case 1: smd_launch_zc<float, 1>(c, a); break;
     case 2: smd_launch_zc<float, 2>(c, a); break;

@@ -5,6 +5,10 @@
 namespace hsmd {
 
 void smd_launch_f64(const SmdLaunch &c, StepArgs a) {
+  if (c.wide) {
+    smd_launch_wide<double>(c, a);
+    return;
+  }
   switch (c.spt) {
     case 1: smd_launch_zc<double, 1>(c, a); break;
     case 2: smd_launch_zc<double, 2>(c, a); break;

@@ -66,6 +66,8 @@ struct StepArgs {
   double *eref;         // per sample, energy at the current time
   void *stage;          // per sample [3][Npad] stage buffers B1, B2 (global-memory form)
   long long stage_stride;   // elements from B1 to B2
+  void *ks;             // wide form: per sample [6][Npad] K1 and K2+K3
+  double *dd;           // wide form: per sample [3][Npad] correction directions
   double dt;
   int t0;               // global time index of the first point of this launch
   int n_points;         // time points produced by this launch
@@ -436,14 +438,203 @@ __global__ __launch_bounds__(kSmdThreads) void smd_energy_kernel(const LatDev *l
   if (threadIdx.x == 0) E[blockIdx.x] = v[0];
 }
 
+// Wide form for lattices beyond the LDS: one workgroup per sample, every
+// array (S, B1, B2, the RK increments K1 and K2 + K3, the correction
+// directions) in global memory, sites visited with a workgroup-stride loop;
+// the same arithmetic, in the same order, as smd_kernel.  All samples run in
+// one launch; the working set (tens of MB at N ~ 10^4) stays in the L2 /
+// Infinity Cache across the time loop.
+template <typename T, int ZC, bool GEN>
+__global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a) {
+  __shared__ double scratch[4 * 16];
+  constexpr int ZH = (ZC + 1) / 2;
+  const int sidx = blockIdx.x + a.sample_base;
+  const SampleDev sd = a.samples[sidx];
+  const LatDev &L = a.lats[sd.li];
+  const int N = sd.N, Npad = N + 1;
+  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L.cp, (const T *)L.Jt);
+  const uint32_t *nb16 = L.nb16;
+  const bool fe = L.field_energy != 0;
+  const double hext = L.cp.h_on ? L.cp.h : 0.0;
+  const int Z = fc.Z;
+  const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
+  T *S = (T *)a.spins + sd.spin_off;
+  const T *g0 = (const T *)a.s0 + sd.spin_off;
+  T *B1 = (T *)a.stage + sd.spin_off;
+  T *B2 = (T *)a.stage + a.stage_stride + sd.spin_off;
+  T *K = (T *)a.ks + 2 * sd.spin_off;
+  double *D = a.dd + sd.spin_off;
+  T *Sx = S, *Sy = S + Npad, *Sz = S + 2 * Npad;
+  T *Px = B1, *Py = B1 + Npad, *Pz = B1 + 2 * Npad;
+  T *Qx = B2, *Qy = B2 + Npad, *Qz = B2 + 2 * Npad;
+  const T dt = T(a.dt);
+  double Eref = a.eref[sidx];
+  T *traj = (T *)a.traj + sd.traj_off;
+  auto field = [&](int p, const T *fx, const T *fy, const T *fz, T &hx, T &hy, T &hz) {
+    uint32_t w[ZH];
+    int pp = p;
+    asm volatile("" : "+v"(pp));
+#pragma unroll
+    for (int k = 0; k < ZH; k++) w[k] = (2 * k < Z) ? nb16[(size_t)k * N + pp] : pad;
+    field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
+  };
+  double qss_prev = 0.0;
+  int t_prev = -1;
+  for (int it = 0; it < a.n_points; it++) {
+    const int t = a.t0 + it;
+    if (t > 0) {
+      double eref_part = 0.0;
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        T hx, hy, hz;
+        field(p, Sx, Sy, Sz, hx, hy, hz);
+        const T x = Sx[p], y = Sy[p], z = Sz[p];
+        if (fe) eref_part += 0.5 * ((double)x * hx + (double)y * hy + (double)z * hz) - 0.5 * hext * z;
+        T px, py, pz;
+        cross3(hx, hy, hz, x, y, z, px, py, pz);
+        const T k1[3] = {dt * px, dt * py, dt * pz};
+        K[p] = k1[0]; K[Npad + p] = k1[1]; K[2 * Npad + p] = k1[2];
+        Px[p] = x + T(0.5) * k1[0];
+        Py[p] = y + T(0.5) * k1[1];
+        Pz[p] = z + T(0.5) * k1[2];
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        T hx, hy, hz;
+        field(p, Px, Py, Pz, hx, hy, hz);
+        T px, py, pz;
+        cross3(hx, hy, hz, Px[p], Py[p], Pz[p], px, py, pz);
+        const T k2[3] = {dt * px, dt * py, dt * pz};
+        K[3 * Npad + p] = k2[0]; K[4 * Npad + p] = k2[1]; K[5 * Npad + p] = k2[2];
+        Qx[p] = Sx[p] + T(0.5) * k2[0];
+        Qy[p] = Sy[p] + T(0.5) * k2[1];
+        Qz[p] = Sz[p] + T(0.5) * k2[2];
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        T hx, hy, hz;
+        field(p, Qx, Qy, Qz, hx, hy, hz);
+        T px, py, pz;
+        cross3(hx, hy, hz, Qx[p], Qy[p], Qz[p], px, py, pz);
+        const T k3[3] = {dt * px, dt * py, dt * pz};
+        Px[p] = Sx[p] + k3[0];
+        Py[p] = Sy[p] + k3[1];
+        Pz[p] = Sz[p] + k3[2];
+        K[3 * Npad + p] += k3[0]; K[4 * Npad + p] += k3[1]; K[5 * Npad + p] += k3[2];
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        T hx, hy, hz;
+        field(p, Px, Py, Pz, hx, hy, hz);
+        T px, py, pz;
+        cross3(hx, hy, hz, Px[p], Py[p], Pz[p], px, py, pz);
+        const T f1 = T(1.0 / 6.0), f2 = T(2.0 / 6.0);
+        const T k4[3] = {dt * px, dt * py, dt * pz};
+        const T s[3] = {Sx[p], Sy[p], Sz[p]};
+        T v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = s[d] + (f1 * (K[d * Npad + p] + k4[d]) + f2 * K[(3 + d) * Npad + p]);
+        const T inrm = T(1) / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        Sx[p] = v[0] * inrm; Sy[p] = v[1] * inrm; Sz[p] = v[2] * inrm;
+      }
+      __syncthreads();
+      double v4[4] = {0.0, 0.0, eref_part, qss_prev};
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        T hx, hy, hz;
+        field(p, Sx, Sy, Sz, hx, hy, hz);
+        const double x = Sx[p], y = Sy[p], z = Sz[p];
+        if (fe) v4[1] += 0.5 * (x * hx + y * hy + z * hz) - 0.5 * hext * z;
+        double px, py, pz, dx, dy, dz;
+        cross3<double>(hx, hy, hz, x, y, z, px, py, pz);
+        v4[0] += px * px + py * py + pz * pz;
+        cross3<double>(x, y, z, px, py, pz, dx, dy, dz);
+        D[p] = dx; D[Npad + p] = dy; D[2 * Npad + p] = dz;
+      }
+      if (!fe) v4[1] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
+      block_sum<4>(v4, scratch);
+      if (fe) Eref = v4[2];
+      if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v4[3] / (double)N;
+      const double cfac = (Eref - v4[1]) / v4[0];
+      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * 3 * N : nullptr;
+      double qs = 0.0;
+      for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+        double x = (double)Sx[p] + cfac * D[p];
+        double y = (double)Sy[p] + cfac * D[Npad + p];
+        double z = (double)Sz[p] + cfac * D[2 * Npad + p];
+        const double nr = sqrt(x * x + y * y + z * z);
+        x /= nr; y /= nr; z /= nr;
+        qs += (double)g0[p] * x + (double)g0[Npad + p] * y + (double)g0[2 * Npad + p] * z;
+        const T tx = (T)x, ty = (T)y, tz = (T)z;
+        Sx[p] = tx; Sy[p] = ty; Sz[p] = tz;
+        if (row) { row[p] = tx; row[N + p] = ty; row[2 * N + p] = tz; }
+      }
+      __syncthreads();
+      qss_prev = qs;
+      t_prev = t;
+      if (!fe) {
+        double v1[1] = {bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz)};
+        block_sum<1>(v1, scratch);
+        Eref = v1[0];
+      }
+    } else {
+      if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + 0] = 1.0;
+      if (a.traj) {
+        T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
+        for (int p = threadIdx.x; p < N; p += blockDim.x) {
+          row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
+        }
+      }
+    }
+  }
+  double e_part = 0.0;
+  if (fe) {
+    for (int p = threadIdx.x; p < N; p += kSmdThreads) {
+      T hx, hy, hz;
+      field(p, Sx, Sy, Sz, hx, hy, hz);
+      const double x = Sx[p], y = Sy[p], z = Sz[p];
+      e_part += 0.5 * (x * hx + y * hy + z * hz) - 0.5 * hext * z;
+    }
+  }
+  double v2[2] = {e_part, qss_prev};
+  block_sum<2>(v2, scratch);
+  if (fe) Eref = v2[0];
+  if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v2[1] / (double)N;
+  if (threadIdx.x == 0) a.eref[sidx] = Eref;
+}
+
 struct SmdLaunch {
   int S;
   int spt;
   int maxZ;
   bool general;   // some lattice needs per-slot J tables
   bool use_lds;
+  bool wide;      // global-memory wide form
+  bool plain;     // one sample per launch (the reference's one instance per rank)
   size_t lds_bytes;
 };
+
+template <typename T, int ZC, bool GEN>
+void smd_launch_wide_t(const SmdLaunch &c, StepArgs a) {
+  if (c.plain) {
+    for (int s = 0; s < c.S; s++) {
+      a.sample_base = s;
+      hipLaunchKernelGGL((smd_kernel_wide<T, ZC, GEN>), dim3(1), dim3(kSmdThreads), 0, 0, a);
+    }
+  } else {
+    a.sample_base = 0;
+    hipLaunchKernelGGL((smd_kernel_wide<T, ZC, GEN>), dim3(c.S), dim3(kSmdThreads), 0, 0, a);
+  }
+}
+
+template <typename T>
+void smd_launch_wide(const SmdLaunch &c, const StepArgs &a) {
+  const int Z = c.maxZ;
+  if (c.general) smd_launch_wide_t<T, 32, true>(c, a);
+  else if (Z <= 4) smd_launch_wide_t<T, 4, false>(c, a);
+  else if (Z <= 8) smd_launch_wide_t<T, 8, false>(c, a);
+  else if (Z <= 12) smd_launch_wide_t<T, 12, false>(c, a);
+  else if (Z <= 20) smd_launch_wide_t<T, 20, false>(c, a);
+  else smd_launch_wide_t<T, 32, false>(c, a);
+}
 
 // LDS-resident form: S, B1, B2 of one sample fit 150 KB only for N <= 4096
 // (FP32) or N <= 2048 (FP64), i.e. SPT x sizeof(T) <= 16.

@@ -242,3 +242,24 @@ def test_measured_order_parameters_match_oracle(geom, L, kw, model):
         np.testing.assert_allclose(m[16], x, rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose([m[3], m[8], m[11], m[17]], [F1 ** 2, Q1 ** 2, S1 ** 2, X1 ** 2],
                                    rtol=1e-10, atol=1e-12)
+
+
+def test_poisson_disk_graph_fp64_sweep_matches_colour_replay():
+    # MC on an irregular tabulated graph: one FP64 sweep replayed on the CPU
+    _say("poisson graph mc replay")
+    box = 30.0
+    pts = H.poisson_disk(box, 1.0, seed=7)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
+    lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
+    R = 4
+    temps = np.geomspace(0.1, 1.0, R)
+    mc = H.MonteCarlo(lat, temps, precision=H.PLAIN_F64, n_or=2, n_hb=1, seed=21)
+    rng = np.random.default_rng(22)
+    s = _spins(rng, R, lat.N)
+    mc.set_spins(s)
+    mc.run(1, pt_parity=-1)
+    got = mc.get_spins()
+    m = _model(lat, c)
+    ref = s.copy()
+    m.mc_replay(0, temps, lat.colours(), ref, 1, n_or=2, n_hb=1, seed=21)
+    np.testing.assert_allclose(got, ref, atol=1e-9)

@@ -196,3 +196,50 @@ def test_dssf_q_on_a_periodic_lattice_with_vacancies_matches_oracle():
         _, tr, _, _ = m.smd_run(samples[k], 0.02, T, gauss_seidel=False)
         ref += cpu_ref.dssf_direct(tr, rv, path, w, no, mask=vac)
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-10 * ref.max())
+
+
+@pytest.mark.parametrize("prec,tol", [(H.BATCHED_F64, 1e-10), (H.BATCHED_F32, 2e-5)])
+def test_wide_form_beyond_lds_matches_oracle(prec, tol):
+    # N = 5184 exceeds the LDS-resident form: every array lives in global memory
+    _say("wide form", prec)
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 72, c)
+    m = _model(lat, c)
+    rng = np.random.default_rng(14)
+    samples = [_spins(rng, lat.N) for _ in range(2)]
+    T = 41
+    smd = H.SpinDynamics([lat], [0, 0], 0.01, T, n_omega=32, precision=prec, dssf=False)
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    out = smd.get_samples()
+    E = smd.energies()
+    for k in range(2):
+        s_ref, _, _, e = m.smd_run(samples[k], 0.01, T, gauss_seidel=False, traj=False)
+        assert np.abs(out[k] - s_ref).max() < tol, k
+        assert E[k] == pytest.approx(e[-1], rel=max(tol, 1e-9))
+
+
+def test_poisson_disk_graph_with_vacancies_matches_oracle():
+    # an irregular tabulated graph (distance shells, 5% vacancies)
+    _say("poisson graph smd")
+    box = 40.0
+    pts = H.poisson_disk(box, 1.0, seed=5)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
+    lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
+    rng = np.random.default_rng(15)
+    vac = (rng.random(lat.N) < 0.05).astype(np.int32)
+    lat.set_vacancies(vac)
+    nb = lat.tables().copy()
+    for v in np.nonzero(vac)[0]:
+        nb[v, :] = -1
+        nb[nb == v] = -1
+    m = cpu_ref.Model(nb, (lat.zn1, lat.zn2, lat.znx), c.as_params())
+    samples = [_spins(rng, lat.N) for _ in range(3)]
+    T = 51
+    smd = H.SpinDynamics([lat], [0, 0, 0], 0.01, T, n_omega=32, precision=H.BATCHED_F64, dssf=False)
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    out = smd.get_samples()
+    for k in range(3):
+        s_ref = m.smd_run(samples[k], 0.01, T, gauss_seidel=False, traj=False)[0]
+        assert np.abs(out[k] - s_ref).max() < 1e-10, k

@@ -18,6 +18,11 @@ Also reported on the same line:
                     L=24 square lattice with 32 samples;
   roofline       -- the sweep kernel's algorithmic bytes per launch over its
                     average launch duration (HIP events on the launch stream);
+  graph          -- irregular graph: about 12,000 seeded Poisson-disk points
+                    (r_min = 1) in a periodic box, shells J1=1 (r<1.15),
+                    J2=0.4 (<1.45), JX=0.1 (<1.75), 5 % vacancies; MC with
+                    128 FP32 replicas, SMD with 256 samples over 4096 RK4
+                    steps (dt=0.01) and the DSSF along a 200-point q-path;
   cpu_baseline   -- the oracle's reference-order FP64 sweeps (one thread per
                     replica, like one MPI rank each) on a bounded sample.
 """
@@ -115,6 +120,77 @@ def smd_workload(rng):
     return lats, idx, samples
 
 
+def graph_workload(rng, n_target=12000):
+    """Poisson-disk graph of about n_target sites (Bridson sampling fills a
+    periodic box at ~0.627 points per r_min^2) with 5 % vacancies."""
+    box = math.sqrt(n_target / 0.627)
+    pts = H.poisson_disk(box, 1.0, seed=5)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
+    lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
+    vac = np.zeros(lat.N, np.int32)
+    vac[rng.choice(lat.N, int(round(0.05 * lat.N)), replace=False)] = 1
+    lat.set_vacancies(vac)
+    return c, lat, box
+
+
+def q_path(n=200, qmax=2 * math.pi):
+    """Gamma -> (qmax, 0) -> (qmax, qmax) -> Gamma, n points by arc length."""
+    corners = np.array([[0.0, 0.0], [qmax, 0.0], [qmax, qmax], [0.0, 0.0]])
+    seg = np.linalg.norm(np.diff(corners, axis=0), axis=1)
+    s = np.linspace(0.0, seg.sum(), n)
+    out = np.zeros((n, 2))
+    for k, v in enumerate(s):
+        i = min(np.searchsorted(np.cumsum(seg), v, side="right"), 2)
+        t = (v - (np.cumsum(seg)[i - 1] if i > 0 else 0.0)) / seg[i]
+        out[k] = corners[i] + min(max(t, 0.0), 1.0) * (corners[i + 1] - corners[i])
+    return out
+
+
+def bench_graph(timer, rng, mc_steps=50, smd_steps=1024):
+    c, lat, box = graph_workload(rng)
+    N = lat.N
+    print("[bench] graph: N=%d box=%.1f Z=%d colours=%d" % (N, box, lat.zn1 + lat.zn2 + lat.znx,
+                                                            lat.n_colours), file=sys.stderr, flush=True)
+    temps = np.geomspace(0.05, 1.5, 128)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, n_or_boost=5,
+                      n_hb_boost=1, pt_cap50=False, seed=5)
+    mc.set_spins(unit_spins(rng, (128, N, 3)))
+    mc.run(5, pt_parity=2)
+    timer.start()
+    mc.run(mc_steps, pt_parity=2)
+    mms = timer.stop()
+    mrate = 128 * N * 6 * mc_steps / (mms * 1e-3)
+    del mc
+    S, T = 256, 4096
+    smd = H.SpinDynamics([lat], np.zeros(S, np.int32), 0.01, T, n_omega=512,
+                         precision=H.BATCHED_F32)
+    smd.set_samples([unit_spins(rng, (N, 3)) for _ in range(S)])
+    smd.step(1023)
+    timer.start()
+    smd.step(smd_steps)
+    sms = timer.stop()
+    srate = S * N * smd_steps / (sms * 1e-3)
+    smd.step(T - 1 - 1023 - smd_steps)
+    q = q_path(200)
+    timer.start()
+    d = smd.dssf_q(q)
+    qms = timer.stop()
+    dflop = 8.0 * 3 * len(q) * N * 512 * S
+    print("[bench] graph MC f32: %.3f ms/sweep %.3e spin-updates/s; SMD f32: %.3f ms/step "
+          "%.3e site-steps/s; q-path DSSF %.1f ms" % (mms / mc_steps, mrate, sms / smd_steps, srate,
+                                                     qms), file=sys.stderr, flush=True)
+    return {"workload": "Poisson-disk graph N=%d (r_min=1, periodic box %.1f), J1=1 r<1.15, "
+                        "J2=0.4 r<1.45, JX=0.1 r<1.75, 5%% vacancies" % (N, box),
+            "sites": N, "colours": lat.n_colours,
+            "mc": {"value": mrate, "unit": "spin-updates/s", "replicas": 128,
+                   "ms_per_sweep": mms / mc_steps, "sweeps": mc_steps},
+            "smd": {"value": srate, "unit": "site-RK4-steps/s", "samples": S, "steps": smd_steps,
+                    "ms_per_step": sms / smd_steps, "n_steps_total": T},
+            "dssf_qpath": {"q_points": len(q), "ms": qms,
+                           "tflops": dflop / (qms * 1e-3) / 1e12,
+                           "finite": bool(np.isfinite(d).all())}}
+
+
 def cpu_baseline(lat, c, temps, seconds_target=15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cpu_ref
@@ -186,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -328,6 +405,8 @@ def main():
                                                   "batched_over_plain": brate / prate}
         print("[bench] SMD square24: plain f64 %.3e, batched f32 %.3e" % (prate, brate),
               file=sys.stderr, flush=True)
+    if not args.no_graph:
+        out["graph"] = bench_graph(timer, rng)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, c, temps)
     print(json.dumps(out))

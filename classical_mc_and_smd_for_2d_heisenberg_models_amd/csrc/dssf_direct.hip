@@ -13,9 +13,13 @@
 //
 // Tiling: 256 threads as 16 x 16, each thread a 4 (q) x 4 (bin) block of
 // complex accumulators; K staged through LDS 16 sites at a time.  A
-// workgroup loops over the samples itself, so each output element has one
-// writer and no atomics.
+// workgroup loops over one group of samples; when the q list is short (a
+// q-path) the samples are split into groups so the launch still fills the
+// chip, each group writing its own partial sum, which a second kernel adds
+// in a fixed order (no atomics, deterministic).
 #include <math.h>
+
+#include <algorithm>
 
 #include "smd_kernel.h"
 
@@ -37,12 +41,15 @@ __global__ void phase_table_kernel(int nq, const double *q, int N, const double 
 
 template <typename T>
 __global__ __launch_bounds__(256) void dssf_direct_kernel(const SampleDev *samples, const int *ids,
-                                                          int n_samples, int N, int nq, int nop,
-                                                          const T *P, const T *Y, double scale,
-                                                          double *dssf) {
+                                                          int n_samples, int per_group, int N,
+                                                          int nq, int nop, const T *P, const T *Y,
+                                                          double scale, double *part) {
   __shared__ T Ps[kDK][kDQ][2];
   __shared__ T Ys[kDK][kDB][2];
-  const int q0 = blockIdx.x * kDQ, b0 = blockIdx.y * kDB, d = blockIdx.z;
+  const int q0 = blockIdx.x * kDQ, b0 = blockIdx.y * kDB;
+  const int d = blockIdx.z % 3, grp = blockIdx.z / 3;
+  const int s_begin = grp * per_group, s_end = min(n_samples, s_begin + per_group);
+  double *dssf = part + (size_t)grp * 3 * nq * nop;
   const int tq = threadIdx.x >> 4, tb = threadIdx.x & 15;
   const int ldy = 2 * nop;
   double acc[4][4];
@@ -50,7 +57,7 @@ __global__ __launch_bounds__(256) void dssf_direct_kernel(const SampleDev *sampl
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-  for (int sl = 0; sl < n_samples; sl++) {
+  for (int sl = s_begin; sl < s_end; sl++) {
     const SampleDev sd = samples[ids[sl]];
     const T *Yb = Y + sd.y_off + (size_t)d * N * ldy;
     T xr[4][4], xi[4][4];
@@ -114,29 +121,51 @@ __global__ __launch_bounds__(256) void dssf_direct_kernel(const SampleDev *sampl
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int n = b0 + tb * 4 + j;
-      if (n < nop) dssf[((size_t)d * nq + q) * nop + n] += scale * acc[i][j];
+      if (n < nop) dssf[((size_t)d * nq + q) * nop + n] = scale * acc[i][j];
     }
   }
+}
+
+// out[i] += sum over groups g of part[g][i], groups in order
+__global__ void dssf_group_sum_kernel(int groups, size_t n, const double *part, double *out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int g = 0; g < groups; g++) s += part[(size_t)g * n + i];
+  out[i] += s;
+}
+
+// part: scratch of groups x 3 x nq x nop doubles (groups from
+// dssf_direct_groups); dssf accumulates the sample sums.
+int dssf_direct_groups(int n_samples, int nq, int nop) {
+  const int tiles = ((nq + kDQ - 1) / kDQ) * ((nop + kDB - 1) / kDB) * 3;
+  const int want = (2048 + tiles - 1) / tiles;
+  return std::max(1, std::min(n_samples, std::min(want, 64)));
 }
 
 template <typename T>
 void dssf_direct_launch(const SampleDev *samples, const int *ids, int n_samples, int N, int nq,
                         int nop, const double *q, const double *pos, const uint8_t *mask, T *P,
-                        const T *Y, double scale, double *dssf) {
+                        const T *Y, double scale, int groups, double *part, double *dssf) {
   const size_t np = (size_t)nq * N;
   hipLaunchKernelGGL(phase_table_kernel<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, 0, nq,
                      q, N, pos, mask, P);
-  dim3 grid((nq + kDQ - 1) / kDQ, (nop + kDB - 1) / kDB, 3);
-  hipLaunchKernelGGL(dssf_direct_kernel<T>, grid, dim3(256), 0, 0, samples, ids, n_samples, N, nq,
-                     nop, P, Y, scale, dssf);
+  const int per_group = (n_samples + groups - 1) / groups;
+  const int used = (n_samples + per_group - 1) / per_group;
+  dim3 grid((nq + kDQ - 1) / kDQ, (nop + kDB - 1) / kDB, 3 * used);
+  hipLaunchKernelGGL(dssf_direct_kernel<T>, grid, dim3(256), 0, 0, samples, ids, n_samples,
+                     per_group, N, nq, nop, P, Y, scale, part);
+  const size_t n = (size_t)3 * nq * nop;
+  hipLaunchKernelGGL(dssf_group_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, used,
+                     n, (const double *)part, dssf);
   HIP_CHECK(hipGetLastError());
 }
 
 template void dssf_direct_launch<float>(const SampleDev *, const int *, int, int, int, int,
                                         const double *, const double *, const uint8_t *, float *,
-                                        const float *, double, double *);
+                                        const float *, double, int, double *, double *);
 template void dssf_direct_launch<double>(const SampleDev *, const int *, int, int, int, int,
                                          const double *, const double *, const uint8_t *, double *,
-                                         const double *, double, double *);
+                                         const double *, double, int, double *, double *);
 
 }  // namespace hsmd

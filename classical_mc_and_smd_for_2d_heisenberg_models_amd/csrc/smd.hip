@@ -505,10 +505,11 @@ void smd_step(SMDEngine &e, int n) {
   HIP_CHECK(hipDeviceSynchronize());
 }
 
+int dssf_direct_groups(int n_samples, int nq, int nop);
 template <typename T>
 void dssf_direct_launch(const SampleDev *samples, const int *ids, int n_samples, int N, int nq,
                         int nop, const double *q, const double *pos, const uint8_t *mask, T *P,
-                        const T *Y, double scale, double *dssf);
+                        const T *Y, double scale, int groups, double *part, double *dssf);
 
 // DSSF at wavevectors q (nq x 2) by direct spatial sums over the sites of
 // lattice li (vacant sites excluded): out[3][nq][n_omega], sample-summed
@@ -539,6 +540,7 @@ static void dssf_direct(SMDEngine &e, int li, int nq, const double *q, double *o
   DevBuf<T> P;
   P.alloc((size_t)qc * N * 2);
   d.alloc((size_t)3 * qc * no);
+  DevBuf<double> part;   // per-group partial sums, sized per chunk
   const double scale = 1.0 / ((double)N * (double)e.spec.n_steps);
   std::vector<double> h((size_t)3 * qc * no);
   for (int q0 = 0; q0 < nq; q0 += qc) {
@@ -546,8 +548,10 @@ static void dssf_direct(SMDEngine &e, int li, int nq, const double *q, double *o
     d.zero();
     if (!ids.empty()) {
       dq.upload(q + 2 * (size_t)q0, (size_t)2 * nc);
+      const int groups = dssf_direct_groups((int)ids.size(), nc, no);
+      if (part.n < (size_t)groups * 3 * nc * no) part.alloc((size_t)groups * 3 * nc * no);
       dssf_direct_launch<T>(e.dsamples.p, dids.p, (int)ids.size(), N, nc, no, dq.p, dpos.p,
-                            dmask.p, P.p, (const T *)e.ybuf.p, scale, d.p);
+                            dmask.p, P.p, (const T *)e.ybuf.p, scale, groups, part.p, d.p);
     }
     HIP_CHECK(hipMemcpy(h.data(), d.p, (size_t)3 * nc * no * sizeof(double), hipMemcpyDeviceToHost));
     for (int c = 0; c < 3; c++)

@@ -30,7 +30,8 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 struct GemmSample {
   int M;                 // rows of Y (3 N)
-  long long a_off;       // into A (chunk, [K][M])
+  int lda;               // row stride of A: M rounded up to a multiple of 4
+  long long a_off;       // into A (chunk, [K][lda])
   long long y_off;       // into Y ([M][ldb])
 };
 
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void time_gemm_mfma_kernel(const GemmSample *s
       const int k = e >> 6, col = e & 63;
       const int kg = kk + k;
       const int m = m0 + col;
-      const float av = (kg < K && m < M) ? Ab[(size_t)kg * M + m] : 0.f;
+      const float av = (kg < K && m < M) ? Ab[(size_t)kg * sd.lda + m] : 0.f;
       const float bv = (kg < K) ? Bm[(size_t)(k0 + kg) * ldb + j0 + col] : 0.f;
       const _Float16 ah = (_Float16)av;
       const _Float16 bh = (_Float16)bv;
@@ -128,7 +129,14 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 // arrive pre-split (FP16 hi/lo, k contiguous per column) and the gfx950
 // v_mfma_f32_16x16x32_f16 consumes 32 time points per instruction.  Each wave
 // owns a 64 x 64 sub-tile (4 x 4 MFMA blocks).
+//
+// Software pipeline: the global loads of step k+1 (two float4 of the
+// trajectory and four 16-byte twiddle vectors per thread) are issued into
+// registers before the MFMAs of step k, so their latency hides behind the
+// matrix work; each thread converts a pair of time points for four rows and
+// writes them to LDS as packed half2, k contiguous per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 constexpr int WM = 128, WJ = 256, WK = 32, WKP = WK + 8, WT = 512;
 
@@ -138,7 +146,7 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
   __shared__ __align__(16) _Float16 Ah[WM * WKP], Al[WM * WKP];
   __shared__ __align__(16) _Float16 Bh[WJ * WKP], Bl[WJ * WKP];
   const GemmSample sd = samples[blockIdx.z];
-  const int M = sd.M;
+  const int M = sd.M, lda = sd.lda;
   const int m0 = blockIdx.x * WM, j0 = blockIdx.y * WJ;
   if (m0 >= M) return;
   const float *Ab = A + sd.a_off;
@@ -146,35 +154,48 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave >> 2) * 64, wj = (wave & 3) * 64;
   const int li = lane & 15, lk = (lane >> 4) * 8;
+  // staging roles: A, a pair of time points (2 kp, 2 kp + 1) for rows
+  // m4 .. m4 + 3; B, half a column (16 time points) of column bc
+  const int kp = tid >> 5, m4 = (tid & 31) * 4;
+  const bool arow = m0 + m4 < M;
+  const int bc = tid >> 1, bh16 = (tid & 1) * 16;
+  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + bh16);
   float4_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  float4_t ra0, ra1;
+  uint4 rbh0, rbh1, rbl0, rbl1;
+  auto fetch = [&](int kk) {
+    const int ka = kk + 2 * kp;
+    const float4_t z = (float4_t){0.f, 0.f, 0.f, 0.f};
+    ra0 = (arow && ka < K) ? *(const float4_t *)(Ab + (size_t)ka * lda + m0 + m4) : z;
+    ra1 = (arow && ka + 1 < K) ? *(const float4_t *)(Ab + (size_t)(ka + 1) * lda + m0 + m4) : z;
+    const uint4 *h = (const uint4 *)(Bth + bsrc + kk);
+    const uint4 *l = (const uint4 *)(Btl + bsrc + kk);
+    rbh0 = h[0]; rbh1 = h[1];
+    rbl0 = l[0]; rbl1 = l[1];
+  };
+  fetch(0);
   for (int kk = 0; kk < K; kk += WK) {
-    // A: 32 (k) x 128 (m) FP32 -> hi/lo halves, [m][k]
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int e = tid + r * WT;
-      const int k = e >> 7, col = e & 127;
-      const int kg = kk + k, m = m0 + col;
-      const float av = (kg < K && m < M) ? Ab[(size_t)kg * M + m] : 0.f;
-      const _Float16 ah = (_Float16)av;
-      Ah[col * WKP + k] = ah;
-      Al[col * WKP + k] = (_Float16)(av - (float)ah);
+    for (int i = 0; i < 4; i++) {
+      const float x0 = ra0[i], x1 = ra1[i];
+      const _Float16 h0 = (_Float16)x0, h1 = (_Float16)x1;
+      const half2_t hh = {h0, h1};
+      const half2_t ll = {(_Float16)(x0 - (float)h0), (_Float16)(x1 - (float)h1)};
+      *(half2_t *)&Ah[(m4 + i) * WKP + 2 * kp] = hh;
+      *(half2_t *)&Al[(m4 + i) * WKP + 2 * kp] = ll;
     }
-    // B: 256 columns x 32 time points, half a column (16 points) per thread
     {
-      const int col = tid >> 1, half = tid & 1;
-      const size_t src = (size_t)(j0 + col) * ldt + (size_t)(k0 + kk + 16 * half);
-      const uint4 *h = (const uint4 *)(Bth + src);
-      const uint4 *l = (const uint4 *)(Btl + src);
-      uint4 *dh = (uint4 *)&Bh[col * WKP + 16 * half];
-      uint4 *dl = (uint4 *)&Bl[col * WKP + 16 * half];
-      dh[0] = h[0]; dh[1] = h[1];
-      dl[0] = l[0]; dl[1] = l[1];
+      uint4 *dh = (uint4 *)&Bh[bc * WKP + bh16];
+      uint4 *dl = (uint4 *)&Bl[bc * WKP + bh16];
+      dh[0] = rbh0; dh[1] = rbh1;
+      dl[0] = rbl0; dl[1] = rbl1;
     }
     __syncthreads();
+    if (kk + WK < K) fetch(kk + WK);
     half8_t ah[4], al[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {

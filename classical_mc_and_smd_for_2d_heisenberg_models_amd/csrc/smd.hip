@@ -48,6 +48,7 @@ __global__ __launch_bounds__(256) void time_gemm_kernel(const SampleDev *samples
   const int m0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
   if (m0 >= M) return;
   const T *Ab = A + sd.traj_off;
+  const int lda = sd.ldm;
   T *Yb = Y + sd.y_off;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   T acc[4][4];
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(256) void time_gemm_kernel(const SampleDev *samples
       const int r = e >> 6, cidx = e & 63;
       const int k = kk + r;
       const int m = m0 + cidx;
-      As[r][cidx] = (k < K && m < M) ? Ab[(size_t)k * M + m] : T(0);
+      As[r][cidx] = (k < K && m < M) ? Ab[(size_t)k * lda + m] : T(0);
       Bs[r][cidx] = (k < K) ? Bm[(size_t)(k0 + k) * ldb + j0 + cidx] : T(0);
     }
     __syncthreads();
@@ -161,6 +162,7 @@ void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const 
 
 struct GemmSampleHost {
   int M;
+  int lda;
   long long a_off;
   long long y_off;
 };
@@ -340,15 +342,17 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->hs.resize(S);
   for (int s = 0; s < S; s++) {
     const int N = e->lats[sample_lattice[s]].N;
-    e->hs[s] = {sample_lattice[s], N, so, to, yo};
+    const int ldm = ((3 * N + 3) / 4) * 4;
+    e->hs[s] = {sample_lattice[s], N, so, to, yo, ldm};
     so += 3LL * (N + 1);
-    to += (long long)e->Tc * 3 * N;
+    to += (long long)e->Tc * ldm;
     yo += 3LL * N * 2 * no;
   }
   e->dsamples.upload(e->hs.data(), S);
   {
     std::vector<GemmSampleHost> gs(S);
-    for (int s = 0; s < S; s++) gs[s] = {3 * e->hs[s].N, e->hs[s].traj_off, e->hs[s].y_off};
+    for (int s = 0; s < S; s++)
+      gs[s] = {3 * e->hs[s].N, e->hs[s].ldm, e->hs[s].traj_off, e->hs[s].y_off};
     e->gemm_samples.upload(gs.data(), S);
   }
   e->spins.alloc(so * es);
@@ -359,6 +363,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->eref.alloc(S);
   if (sp.dssf) {
     e->traj.alloc(to * es);
+    e->traj.zero();   // the pad columns of each row stay zero
     e->ybuf.alloc(yo * es);
     e->ybuf.zero();
     if (e->f32) upload_twiddles<float>(*e); else upload_twiddles<double>(*e);

@@ -24,8 +24,9 @@ struct SampleDev {
   int li;
   int N;
   long long spin_off;   // into spins / S0 (3 * Npad per sample)
-  long long traj_off;   // into trajectory chunk (Tc * 3 * N per sample)
+  long long traj_off;   // into trajectory chunk (Tc * ldm per sample)
   long long y_off;      // into Y accumulators (3 * N * 2 * n_omega per sample)
+  int ldm;              // trajectory row stride: 3 N rounded up to a multiple of 4
 };
 
 constexpr int kSmdThreads = 1024;
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
       if (fe) Eref = v4[2];
       if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v4[3] / (double)N;
       const double cfac = (Eref - v4[1]) / v4[0];
-      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * 3 * N : nullptr;
+      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * sd.ldm : nullptr;
       double qs = 0.0;
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
     } else {
       if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + 0] = 1.0;
       if (a.traj) {
-        T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
+        T *row = traj + (size_t)(t - a.chunk_base) * sd.ldm;
         for (int p = threadIdx.x; p < N; p += blockDim.x) {
           row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
         }
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
       if (fe) Eref = v4[2];
       if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v4[3] / (double)N;
       const double cfac = (Eref - v4[1]) / v4[0];
-      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * 3 * N : nullptr;
+      T *row = a.traj ? traj + (size_t)(t - a.chunk_base) * sd.ldm : nullptr;
       double qs = 0.0;
       for (int p = threadIdx.x; p < N; p += kSmdThreads) {
         double x = (double)Sx[p] + cfac * D[p];
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
     } else {
       if (threadIdx.x == 0) a.qss[(size_t)sidx * a.T + 0] = 1.0;
       if (a.traj) {
-        T *row = traj + (size_t)(t - a.chunk_base) * 3 * N;
+        T *row = traj + (size_t)(t - a.chunk_base) * sd.ldm;
         for (int p = threadIdx.x; p < N; p += blockDim.x) {
           row[p] = Sx[p]; row[N + p] = Sy[p]; row[2 * N + p] = Sz[p];
         }

@@ -147,7 +147,13 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
   __shared__ __align__(16) _Float16 Bh[WJ * WKP], Bl[WJ * WKP];
   const GemmSample sd = samples[blockIdx.z];
   const int M = sd.M, lda = sd.lda;
-  const int m0 = blockIdx.x * WM, j0 = blockIdx.y * WJ;
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+  // XCDs, so the nj frequency tiles of one row block are given ids that land
+  // on the same XCD one after another and share the trajectory tile in that
+  // XCD's L2 (the row-block count is padded to a multiple of 8 by the host).
+  const int nj = ldb / WJ;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int m0 = ((slot / nj) * 8 + xcd) * WM, j0 = (slot % nj) * WJ;
   if (m0 >= M) return;
   const float *Ab = A + sd.a_off;
   float *Yb = Y + sd.y_off;
@@ -234,7 +240,8 @@ void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const 
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
                                 int ldb, float *Y, hipStream_t st) {
   if (ldb % WJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 256");
-  dim3 grid((maxM + WM - 1) / WM, ldb / WJ, S);
+  const int nm = (((maxM + WM - 1) / WM) + 7) / 8 * 8;
+  dim3 grid(nm * (ldb / WJ), 1, S);
   hipLaunchKernelGGL(time_gemm_mfma_wide_kernel, grid, dim3(WT), 0, st,
                      (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
 }

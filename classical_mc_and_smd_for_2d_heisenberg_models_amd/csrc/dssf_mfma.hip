@@ -132,9 +132,10 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 //
 // Software pipeline: the global loads of step k+1 (two float4 of the
 // trajectory and four 16-byte twiddle vectors per thread) are issued into
-// registers before the MFMAs of step k, so their latency hides behind the
-// matrix work; each thread converts a pair of time points for four rows and
-// writes them to LDS as packed half2, k contiguous per row.
+// registers before the MFMAs of step k and written to the second LDS stage
+// after them, so their latency hides behind the matrix work and each step
+// needs one barrier; each thread converts a pair of time points for four
+// rows and writes them to LDS as packed half2, k contiguous per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
@@ -143,8 +144,10 @@ constexpr int WM = 128, WJ = 256, WK = 32, WKP = WK + 8, WT = 512;
 __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
     const GemmSample *samples, const float *__restrict__ A, const _Float16 *__restrict__ Bth,
     const _Float16 *__restrict__ Btl, int ldt, int k0, int K, int ldb, float *Y) {
-  __shared__ __align__(16) _Float16 Ah[WM * WKP], Al[WM * WKP];
-  __shared__ __align__(16) _Float16 Bh[WJ * WKP], Bl[WJ * WKP];
+  // two LDS stages: step k computes from one while step k+1 is written to
+  // the other, one barrier per step
+  __shared__ __align__(16) _Float16 Ah2[2][WM * WKP], Al2[2][WM * WKP];
+  __shared__ __align__(16) _Float16 Bh2[2][WJ * WKP], Bl2[2][WJ * WKP];
   const GemmSample sd = samples[blockIdx.z];
   const int M = sd.M, lda = sd.lda;
   // XCD-aware tile order: workgroups are dispatched round-robin over the 8
@@ -183,8 +186,8 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
     rbh0 = h[0]; rbh1 = h[1];
     rbl0 = l[0]; rbl1 = l[1];
   };
-  fetch(0);
-  for (int kk = 0; kk < K; kk += WK) {
+  auto stage = [&](int buf) {
+    _Float16 *Ah = Ah2[buf], *Al = Al2[buf], *Bh = Bh2[buf], *Bl = Bl2[buf];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const float x0 = ra0[i], x1 = ra1[i];
@@ -194,14 +197,19 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
       *(half2_t *)&Ah[(m4 + i) * WKP + 2 * kp] = hh;
       *(half2_t *)&Al[(m4 + i) * WKP + 2 * kp] = ll;
     }
-    {
-      uint4 *dh = (uint4 *)&Bh[bc * WKP + bh16];
-      uint4 *dl = (uint4 *)&Bl[bc * WKP + bh16];
-      dh[0] = rbh0; dh[1] = rbh1;
-      dl[0] = rbl0; dl[1] = rbl1;
-    }
-    __syncthreads();
-    if (kk + WK < K) fetch(kk + WK);
+    uint4 *dh = (uint4 *)&Bh[bc * WKP + bh16];
+    uint4 *dl = (uint4 *)&Bl[bc * WKP + bh16];
+    dh[0] = rbh0; dh[1] = rbh1;
+    dl[0] = rbl0; dl[1] = rbl1;
+  };
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  for (int kk = 0; kk < K; kk += WK) {
+    const bool more = kk + WK < K;
+    if (more) fetch(kk + WK);
+    const _Float16 *Ah = Ah2[buf], *Al = Al2[buf], *Bh = Bh2[buf], *Bl = Bl2[buf];
     half8_t ah[4], al[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -221,7 +229,9 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
       }
     }
+    if (more) stage(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 4; a++)

@@ -130,12 +130,12 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 // v_mfma_f32_16x16x32_f16 consumes 32 time points per instruction.  Each wave
 // owns a 64 x 64 sub-tile (4 x 4 MFMA blocks).
 //
-// Software pipeline: the global loads of step k+1 (two float4 of the
-// trajectory and four 16-byte twiddle vectors per thread) are issued into
-// registers before the MFMAs of step k and written to the second LDS stage
-// after them, so their latency hides behind the matrix work and each step
-// needs one barrier; each thread converts a pair of time points for four
-// rows and writes them to LDS as packed half2, k contiguous per row.
+// Software pipeline: the global loads of step k+1 (eight trajectory values
+// and four 16-byte twiddle vectors per thread) are issued into registers
+// before the MFMAs of step k and written to the second LDS stage after them,
+// so their latency hides behind the matrix work and each step needs one
+// barrier.  Each thread converts eight time points of one row and writes each
+// half as one 16-byte vector, k contiguous per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
@@ -163,24 +163,27 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave >> 2) * 64, wj = (wave & 3) * 64;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  // staging roles: A, a pair of time points (2 kp, 2 kp + 1) for rows
-  // m4 .. m4 + 3; B, half a column (16 time points) of column bc
-  const int kp = tid >> 5, m4 = (tid & 31) * 4;
-  const bool arow = m0 + m4 < M;
-  const int bc = tid >> 1, bh16 = (tid & 1) * 16;
+  // staging roles, chosen so that the 16 lanes of each LDS write phase touch
+  // 16 consecutive rows (20-dword row stride: conflict-free 16-byte writes):
+  // A, thread t stages time points 8 kg .. 8 kg + 7 of row am = t % 128;
+  // B, thread t stages time points 16 bhalf .. + 15 of column bc = t % 256
+  const int am = tid % WM, kg = tid / WM;
+  const bool arow = m0 + am < M;
+  const int bc = tid % WJ, bh16 = (tid / WJ) * 16;
   const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + bh16);
   float4_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
-  float4_t ra0, ra1;
+  float ra[8];
   uint4 rbh0, rbh1, rbl0, rbl1;
   auto fetch = [&](int kk) {
-    const int ka = kk + 2 * kp;
-    const float4_t z = (float4_t){0.f, 0.f, 0.f, 0.f};
-    ra0 = (arow && ka < K) ? *(const float4_t *)(Ab + (size_t)ka * lda + m0 + m4) : z;
-    ra1 = (arow && ka + 1 < K) ? *(const float4_t *)(Ab + (size_t)(ka + 1) * lda + m0 + m4) : z;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int ka = kk + 8 * kg + e;
+      ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
+    }
     const uint4 *h = (const uint4 *)(Bth + bsrc + kk);
     const uint4 *l = (const uint4 *)(Btl + bsrc + kk);
     rbh0 = h[0]; rbh1 = h[1];
@@ -188,15 +191,15 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
   };
   auto stage = [&](int buf) {
     _Float16 *Ah = Ah2[buf], *Al = Al2[buf], *Bh = Bh2[buf], *Bl = Bl2[buf];
+    half8_t hv, lv;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const float x0 = ra0[i], x1 = ra1[i];
-      const _Float16 h0 = (_Float16)x0, h1 = (_Float16)x1;
-      const half2_t hh = {h0, h1};
-      const half2_t ll = {(_Float16)(x0 - (float)h0), (_Float16)(x1 - (float)h1)};
-      *(half2_t *)&Ah[(m4 + i) * WKP + 2 * kp] = hh;
-      *(half2_t *)&Al[(m4 + i) * WKP + 2 * kp] = ll;
+    for (int e = 0; e < 8; e++) {
+      const _Float16 h = (_Float16)ra[e];
+      hv[e] = h;
+      lv[e] = (_Float16)(ra[e] - (float)h);
     }
+    *(half8_t *)&Ah[am * WKP + 8 * kg] = hv;
+    *(half8_t *)&Al[am * WKP + 8 * kg] = lv;
     uint4 *dh = (uint4 *)&Bh[bc * WKP + bh16];
     uint4 *dl = (uint4 *)&Bl[bc * WKP + bh16];
     dh[0] = rbh0; dh[1] = rbh1;

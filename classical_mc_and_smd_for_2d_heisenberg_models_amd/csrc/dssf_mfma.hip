@@ -13,11 +13,10 @@
 // [-1, 1].  This replaces the reference's per-q MKL FFT over time
 // (CF/subrouts.cpp:12062-12105); the DFT is exact for every bin kept.
 //
-// Tiling: a workgroup of 4 waves owns a 64 (m) x 64 (j) tile of Y; each wave
-// a 32 x 32 quarter as 2 x 2 MFMA blocks.  Per K step of 16 the workgroup
-// stages the FP32 A and B tiles, split into FP16 hi/lo, in LDS with k
-// contiguous so that every lane reads its 4-element fragment as one 8-byte
-// load.
+// Two kernels: time_gemm_mfma_kernel, a 64 x 64 tile with on-the-fly
+// twiddle splitting (v_mfma_f32_16x16x16_f16), kept for bin counts that are
+// not a multiple of 64; and time_gemm_mfma_q_kernel, the main one, described
+// where it is defined.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -123,65 +122,62 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 }
 
 
-// Wide-tile variant for 2 n_omega a multiple of 256: a workgroup of 8 waves
-// owns a 128 (m) x 256 (j) tile, so each trajectory element is staged once
-// per 256 frequency columns and each twiddle once per 128 rows; the twiddles
-// arrive pre-split (FP16 hi/lo, k contiguous per column) and the gfx950
-// v_mfma_f32_16x16x32_f16 consumes 32 time points per instruction.  Each wave
-// owns a 64 x 64 sub-tile (4 x 4 MFMA blocks).
-//
-// Software pipeline: the global loads of step k+1 (eight trajectory values
-// and four 16-byte twiddle vectors per thread) are issued into registers
-// before the MFMAs of step k and written to the second LDS stage after them,
-// so their latency hides behind the matrix work and each step needs one
-// barrier.  Each thread converts eight time points of one row and writes each
-// half as one 16-byte vector, k contiguous per row.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-constexpr int WM = 128, WJ = 256, WK = 32, WKP = WK + 8, WT = 512;
+constexpr int WK = 32, WKP = WK + 8;
 
-__global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
-    const GemmSample *samples, const float *__restrict__ A, const _Float16 *__restrict__ Bth,
-    const _Float16 *__restrict__ Btl, int ldt, int k0, int K, int ldb, float *Y) {
-  // two LDS stages: step k computes from one while step k+1 is written to
-  // the other, one barrier per step
-  __shared__ __align__(16) _Float16 Ah2[2][WM * WKP], Al2[2][WM * WKP];
-  __shared__ __align__(16) _Float16 Bh2[2][WJ * WKP], Bl2[2][WJ * WKP];
+// Main variant (2 n_omega a multiple of 128): a workgroup of 4 waves owns a
+// 128 (m) x 128 (j) tile of Y, each wave a 64 x 64 sub-tile of 4 x 4 MFMA
+// blocks.  The twiddles arrive pre-split (FP16 hi/lo, k contiguous per
+// column) and the gfx950 v_mfma_f32_16x16x32_f16 consumes 32 time points per
+// instruction.  The global loads of step k+1 (sixteen trajectory values and
+// four 16-byte twiddle vectors per thread) are issued into registers before
+// the MFMAs of step k; the single 40 KB LDS stage lets three workgroups share
+// a CU (3 waves per SIMD), so one workgroup's LDS and barrier phases overlap
+// another's matrix work.  Staging roles are chosen so that the 16 lanes of
+// each LDS write phase touch 16 consecutive rows (20-dword row stride:
+// conflict-free 16-byte writes).  Workgroup ids are XCD-aware: workgroups are
+// dispatched round-robin over the 8 XCDs, so the frequency tiles of one row
+// block get ids that land on the same XCD one after another and share the
+// trajectory tile in that XCD's L2 (row-block count padded to a multiple of
+// 8).  Measured on MI355X: 47 ms per 1024-point chunk of the 512-sample
+// batch, 652 TFLOP/s counting the three split products.
+constexpr int QM = 128, QJ = 128, QT = 256;
+
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict__ A,
+                             const _Float16 *__restrict__ Bth, const _Float16 *__restrict__ Btl,
+                             int ldt, int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[QM * WKP], Al[QM * WKP];
+  __shared__ __align__(16) _Float16 Bh[QJ * WKP], Bl[QJ * WKP];
   const GemmSample sd = samples[blockIdx.z];
   const int M = sd.M, lda = sd.lda;
-  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
-  // XCDs, so the nj frequency tiles of one row block are given ids that land
-  // on the same XCD one after another and share the trajectory tile in that
-  // XCD's L2 (the row-block count is padded to a multiple of 8 by the host).
-  const int nj = ldb / WJ;
+  const int nj = ldb / QJ;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int m0 = ((slot / nj) * 8 + xcd) * WM, j0 = (slot % nj) * WJ;
+  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * QJ;
   if (m0 >= M) return;
   const float *Ab = A + sd.a_off;
   float *Yb = Y + sd.y_off;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave >> 2) * 64, wj = (wave & 3) * 64;
+  const int wm = (wave >> 1) * 64, wj = (wave & 1) * 64;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  // staging roles, chosen so that the 16 lanes of each LDS write phase touch
-  // 16 consecutive rows (20-dword row stride: conflict-free 16-byte writes):
-  // A, thread t stages time points 8 kg .. 8 kg + 7 of row am = t % 128;
-  // B, thread t stages time points 16 bhalf .. + 15 of column bc = t % 256
-  const int am = tid % WM, kg = tid / WM;
+  // staging: thread t owns row am = t % 128 (time points 16 kh .. + 15) and
+  // column bc = t % 128 (the same time points); 16 lanes write 16 rows
+  const int am = tid % QM, kh = tid / QM;
   const bool arow = m0 + am < M;
-  const int bc = tid % WJ, bh16 = (tid / WJ) * 16;
-  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + bh16);
+  const int bc = tid % QJ;
+  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + 16 * kh);
   float4_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
-  float ra[8];
+  float ra[16];
   uint4 rbh0, rbh1, rbl0, rbl1;
   auto fetch = [&](int kk) {
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      const int ka = kk + 8 * kg + e;
+    for (int e = 0; e < 16; e++) {
+      const int ka = kk + 16 * kh + e;
       ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
     }
     const uint4 *h = (const uint4 *)(Bth + bsrc + kk);
@@ -189,30 +185,29 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
     rbh0 = h[0]; rbh1 = h[1];
     rbl0 = l[0]; rbl1 = l[1];
   };
-  auto stage = [&](int buf) {
-    _Float16 *Ah = Ah2[buf], *Al = Al2[buf], *Bh = Bh2[buf], *Bl = Bl2[buf];
-    half8_t hv, lv;
+  auto stage = [&]() {
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      const _Float16 h = (_Float16)ra[e];
-      hv[e] = h;
-      lv[e] = (_Float16)(ra[e] - (float)h);
+    for (int g = 0; g < 2; g++) {
+      half8_t hv, lv;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const _Float16 h = (_Float16)ra[8 * g + e];
+        hv[e] = h;
+        lv[e] = (_Float16)(ra[8 * g + e] - (float)h);
+      }
+      *(half8_t *)&Ah[am * WKP + 16 * kh + 8 * g] = hv;
+      *(half8_t *)&Al[am * WKP + 16 * kh + 8 * g] = lv;
     }
-    *(half8_t *)&Ah[am * WKP + 8 * kg] = hv;
-    *(half8_t *)&Al[am * WKP + 8 * kg] = lv;
-    uint4 *dh = (uint4 *)&Bh[bc * WKP + bh16];
-    uint4 *dl = (uint4 *)&Bl[bc * WKP + bh16];
+    uint4 *dh = (uint4 *)&Bh[bc * WKP + 16 * kh];
+    uint4 *dl = (uint4 *)&Bl[bc * WKP + 16 * kh];
     dh[0] = rbh0; dh[1] = rbh1;
     dl[0] = rbl0; dl[1] = rbl1;
   };
   fetch(0);
-  stage(0);
-  __syncthreads();
-  int buf = 0;
   for (int kk = 0; kk < K; kk += WK) {
-    const bool more = kk + WK < K;
-    if (more) fetch(kk + WK);
-    const _Float16 *Ah = Ah2[buf], *Al = Al2[buf], *Bh = Bh2[buf], *Bl = Bl2[buf];
+    stage();
+    __syncthreads();
+    if (kk + WK < K) fetch(kk + WK);
     half8_t ah[4], al[4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -232,9 +227,7 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
       }
     }
-    if (more) stage(buf ^ 1);
     __syncthreads();
-    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 4; a++)
@@ -252,10 +245,10 @@ __global__ __launch_bounds__(WT) void time_gemm_mfma_wide_kernel(
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
                                 int ldb, float *Y, hipStream_t st) {
-  if (ldb % WJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 256");
-  const int nm = (((maxM + WM - 1) / WM) + 7) / 8 * 8;
-  dim3 grid(nm * (ldb / WJ), 1, S);
-  hipLaunchKernelGGL(time_gemm_mfma_wide_kernel, grid, dim3(WT), 0, st,
+  if (ldb % QJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 128");
+  const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
+  dim3 grid(nm * (ldb / QJ), 1, S);
+  hipLaunchKernelGGL(time_gemm_mfma_q_kernel, grid, dim3(QT), 0, st,
                      (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
 }
 

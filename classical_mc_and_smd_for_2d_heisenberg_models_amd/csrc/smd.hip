@@ -240,6 +240,29 @@ static void upload_twiddles(SMDEngine &e) {
   }
 }
 
+// The SMD kernels keep the reference's site order rather than the MC colour
+// order: with it the neighbour gathers of consecutive sites (i +- 1, i +- L,
+// ...) fall on consecutive LDS banks and cache lines, while the colour order
+// scatters them.  Returns L with its device tables re-expressed in that order
+// (identity permutation).
+static Lattice natural_order(const Lattice &L) {
+  Lattice o = L;
+  const int N = L.N, Z = L.Z;
+  for (int s = 0; s < Z; s++)
+    for (int i = 0; i < N; i++) {
+      const int q = L.g_nb[(size_t)s * N + L.inv[i]];
+      o.g_nb[(size_t)s * N + i] = q < N ? L.perm[q] : N;
+    }
+  if (!L.g_J.empty())
+    for (int s = 0; s < Z; s++)
+      for (int i = 0; i < N; i++) o.g_J[(size_t)s * N + i] = L.g_J[(size_t)s * N + L.inv[i]];
+  o.g_bi = L.bi;
+  o.g_bj = L.bj;
+  for (int i = 0; i < N; i++) { o.perm[i] = i; o.inv[i] = i; }
+  o.g_perm = o.perm;
+  return o;
+}
+
 SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
                       const hsmd_smd_spec &sp, const double *window) {
   if (S < 1) throw Error(HSMD_ERR_ARG, "sample count must be >= 1");
@@ -254,7 +277,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->S = S;
   e->f32 = sp.precision == HSMD_BATCHED_F32;
   const size_t es = e->f32 ? sizeof(float) : sizeof(double);
-  for (int i = 0; i < nl; i++) e->lats.push_back(*lats[i]);
+  for (int i = 0; i < nl; i++) e->lats.push_back(natural_order(*lats[i]));
   e->sample_lat.assign(sample_lattice, sample_lattice + S);
   int maxN = 0;
   for (int s = 0; s < S; s++) {

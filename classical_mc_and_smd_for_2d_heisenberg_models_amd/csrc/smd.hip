@@ -176,6 +176,7 @@ struct SMDEngine {
   bool wide = false;
   int spt = 1;
   int maxZ = 0;
+  int minZ = 1 << 30;
   bool general = false;
   size_t lds_bytes = 0;
   int lds_npad = 0;
@@ -294,6 +295,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
     const Lattice &L = e->lats[i];
     if (L.Z > kZMAX) throw Error(HSMD_ERR_ARG, "too many neighbour slots");
     e->maxZ = std::max(e->maxZ, L.Z);
+    e->minZ = std::min(e->minZ, L.Z);
     e->general = e->general || L.general;
     e->nb[i].upload(L.g_nb.data(), L.g_nb.size());
     e->bi[i].upload(L.g_bi.data(), L.g_bi.size());
@@ -521,7 +523,7 @@ void smd_step(SMDEngine &e, int n) {
     a.stage_stride = (long long)(e.stage.n / (2 * (e.f32 ? sizeof(float) : sizeof(double))));
     a.ks = e.ks.p;
     a.dd = e.dd.p;
-    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.general, e.use_lds, e.wide,
+    const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.minZ, e.general, e.use_lds, e.wide,
                           e.spec.precision == HSMD_PLAIN_F64, e.lds_bytes};
     if (e.f32) smd_launch_f32(lc, a); else smd_launch_f64(lc, a);
     HIP_CHECK(hipGetLastError());

@@ -173,9 +173,7 @@ __device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t
 // fields the step computes anyway, E = 1/2 sum_i S_i.h_i - 1/2 h sum_i S_z,i;
 // otherwise from the bond list.
 template <typename T, int SPT, bool LDS, int ZC, bool GEN>
-__global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double scratch[4 * 16];
+__device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem, double *scratch) {
   constexpr int ZH = (ZC + 1) / 2;
   constexpr bool QREG = SPT * ZH <= 16;   // labels in registers
   constexpr bool GREG = SPT <= 2;         // t = 0 spins in registers
@@ -424,6 +422,25 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
   }
 }
 
+template <typename T, int SPT, bool LDS, int ZC, bool GEN>
+__global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[4 * 16];
+  smd_body<T, SPT, LDS, ZC, GEN>(a, smem, scratch);
+}
+
+// Mixed batches whose lattices fall in two slot classes: each workgroup runs
+// the body of its own lattice's class, so a lattice with 8 coupled slots is
+// not summed over the 12 slots another lattice of the batch needs.
+template <typename T, int SPT, int ZLO, int ZHI>
+__global__ __launch_bounds__(kSmdThreads) void smd_kernel_two(const StepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double scratch[4 * 16];
+  const SampleDev sd = a.samples[blockIdx.x + a.sample_base];
+  if (a.lats[sd.li].cp.Z <= ZLO) smd_body<T, SPT, true, ZLO, false>(a, smem, scratch);
+  else smd_body<T, SPT, true, ZHI, false>(a, smem, scratch);
+}
+
 // Energy of every sample (FP64 reduction), used to seed E_ref.
 template <typename T>
 __global__ __launch_bounds__(kSmdThreads) void smd_energy_kernel(const LatDev *lats,
@@ -606,6 +623,7 @@ struct SmdLaunch {
   int S;
   int spt;
   int maxZ;
+  int minZ;
   bool general;   // some lattice needs per-slot J tables
   bool use_lds;
   bool wide;      // global-memory wide form
@@ -662,9 +680,30 @@ void smd_launch_t(const SmdLaunch &c, StepArgs a) {
 // Slot-count classes: 4 (square, kagome, hexagonal and Lieb J1), 8, 12, 20
 // (triangular and kagome up to the third shell) and 32; lattices with per-slot
 // J tables take the 32-slot form with run-time slot tests.
+template <typename T, int SPT, int ZLO, int ZHI>
+void smd_launch_two(const SmdLaunch &c, StepArgs a) {
+  const void *fn = (const void *)smd_kernel_two<T, SPT, ZLO, ZHI>;
+  HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_bytes));
+  a.sample_base = 0;
+  hipLaunchKernelGGL((smd_kernel_two<T, SPT, ZLO, ZHI>), dim3(c.S), dim3(kSmdThreads), c.lds_bytes, 0,
+                     a);
+}
+
 template <typename T, int SPT>
 void smd_launch_zc(const SmdLaunch &c, const StepArgs &a) {
   const int Z = c.maxZ;
+  // two slot classes in one LDS-resident batch (FP32 only)
+  if constexpr (sizeof(T) == 4 && SPT <= 4) {
+    if (c.use_lds && !c.general && c.minZ <= 8 && Z > 8 && Z <= 12) {
+      if (c.minZ <= 4) smd_launch_two<T, SPT, 4, 12>(c, a);
+      else smd_launch_two<T, SPT, 8, 12>(c, a);
+      return;
+    }
+    if (c.use_lds && !c.general && c.minZ <= 4 && Z > 4 && Z <= 8) {
+      smd_launch_two<T, SPT, 4, 8>(c, a);
+      return;
+    }
+  }
   if (c.general) smd_launch_t<T, SPT, 32, true>(c, a);
   else if (Z <= 4) smd_launch_t<T, SPT, 4, false>(c, a);
   else if (Z <= 8) smd_launch_t<T, SPT, 8, false>(c, a);

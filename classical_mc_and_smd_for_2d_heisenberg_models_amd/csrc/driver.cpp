@@ -26,6 +26,13 @@
 
 namespace hsmd {
 
+// postproc.cpp
+void write_mc_spectra(const std::string &dir, const Lattice &L, const double *sq,
+                      const std::string &tagDat);
+void write_smd_spectra(const std::string &dir, const Lattice &L, const double *dssf,
+                       const double *qss, int ntm, double dtm, double Temp,
+                       const std::string &tagDat);
+
 namespace {
 
 // Settings_CodeExec.txt order (Set_Params.py code_exec_labels):
@@ -534,9 +541,20 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
     std::vector<double> sqf((size_t)npt * Nq * 6);
     int64_t cnt = 0;
     check(hsmd_mc_sq_field(mc, sqf.data(), &cnt, 0));
-    for (int k = 0; k < npt; k++)
-      write_w0_sq_field(dir + "/outBin/MC_Bin_w0SqField" + t_tag(T[k], s.label) + ".bin",
-                        &sqf[(size_t)k * Nq * 6], Nq, cnt);
+    for (int k = 0; k < npt; k++) {
+      const double *f = &sqf[(size_t)k * Nq * 6];
+      write_w0_sq_field(dir + "/outBin/MC_Bin_w0SqField" + t_tag(T[k], s.label) + ".bin", f, Nq, cnt);
+      // averaged S(q) (fc (sum |S|) e^{i <arg S>}) -> zone maps and path files
+      std::vector<double> sq((size_t)Nq * 6);
+      const double fc = 1.0 / (double)cnt;
+      for (int q = 0; q < Nq; q++)
+        for (int d = 0; d < 3; d++) {
+          const double amp = fc * f[(size_t)q * 6 + d], ph = fc * f[(size_t)q * 6 + 3 + d];
+          sq[((size_t)q * 3 + d) * 2] = amp * cos(ph);
+          sq[((size_t)q * 3 + d) * 2 + 1] = amp * sin(ph);
+        }
+      write_mc_spectra(dir, lh.p->lat, sq.data(), t_tag(T[k], s.label) + ".dat");
+    }
   }
   write_bin(dir + "/outBin/Temp_List" + s.label + ".bin", T.data(), T.size(), false);
 }
@@ -653,6 +671,10 @@ void run_smd(const std::string &dir, int precision, bool strict) {
       for (int n = 0; n < ntm; n++) out.push_back(fc * acc[((size_t)d * Nq + k) * ntm + n]);
   for (int n = 0; n < ntm; n++) out.push_back(fc * q[n]);
   write_bin(dir + "/outBin/MC_Bin_SpecData" + t_tag(Temp, s.label) + ".bin", out.data(), out.size(), false);
+  // text outputs (quantum correction, path, w-slices, qSS), CF/smd_code.cpp:3628-4130
+  if (s.code[WITH_DFT])
+    write_smd_spectra(dir, lh.p->lat, out.data(), out.data() + (size_t)3 * Nq * ntm, ntm, s.dtm,
+                      Temp, t_tag(Temp, s.label) + ".dat");
 }
 
 }  // namespace hsmd

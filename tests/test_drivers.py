@@ -102,3 +102,39 @@ def test_mc_then_smd_pipeline_files_and_dssf(tmp_path):
     qref /= len(samples)
     np.testing.assert_allclose(S.transpose(1, 0, 2), ref, rtol=1e-7, atol=1e-9 * ref.max())
     np.testing.assert_allclose(qss, qref, rtol=1e-9, atol=1e-12)
+    # text outputs (CF/smd_code.cpp:3628-4130): quantum-corrected spectrum
+    # along the YGMXG path, w-slices, qSS; checked at the Gamma point against
+    # the binary file and the closed-form correction factor
+    label_dat = "_T(%.5f)%s.dat" % (T[1], label)
+    out = os.path.join(d, "outData")
+    cq = np.loadtxt(os.path.join(out, "MC_Results", "SpinSpin_Temporal_Corr" + label_dat))
+    np.testing.assert_allclose(cq[:, 0], np.arange(ntm) * 0.02, rtol=1e-8)
+    np.testing.assert_allclose(cq[:, 1], qss, rtol=1e-11, atol=1e-14)
+    zz = np.loadtxt(os.path.join(out, "RK_Results", "YGMXG_wDepn_DFT_zzSpec" + label_dat))
+    npw, l2 = ntm // 2, 3
+    assert zz.shape == ((4 * l2 + 1) * npw, 3)
+    zz = zz.reshape(4 * l2 + 1, npw, 3)
+    tmax = ntm * 0.02
+    dwf = 2 * np.pi / tmax
+    beta = 1.0 / T[1]
+    n = np.arange(npw)
+    fc = beta * n * dwf
+    qt = np.where(n > 0, fc / (1 - np.exp(-np.where(n > 0, fc, 1.0))), 1.0)
+    # path point l2 is Gamma (FFT index 0); the spectrum there is S_z(q = 0)
+    np.testing.assert_allclose(zz[l2, :, 1], n * dwf, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(zz[l2, :, 2], tmax / (2 * np.pi) * qt * S[0, 2, :npw], rtol=1e-10)
+    slices = [f for f in os.listdir(os.path.join(out, "RK_Results")) if f.startswith("DFT_zzSpec_wfreq(")]
+    assert len(slices) == len(range(0, npw, int(np.ceil(0.1 * npw))))
+    # MC static spectrum: zone map and path at Gamma = |<S_z(0)>|^2 from MC_Bin_w0SqField
+    tag0 = "_T(%.5f)%s" % (T[1], label)
+    raw = open(os.path.join(d, "outBin", "MC_Bin_w0SqField" + tag0 + ".bin"), "rb").read()
+    cnt = np.frombuffer(raw[:4], np.int32)[0]
+    w0 = np.frombuffer(raw[4:], np.complex128).reshape(N, 3) / cnt
+    zmap = np.loadtxt(os.path.join(out, "MC_Results", "DFT_zzSpec" + tag0 + ".dat"))
+    assert zmap.shape == (7 * 7, 3)
+    g0 = zmap[(zmap[:, 0] == 0) & (zmap[:, 1] == 0)]
+    assert g0.shape[0] == 1
+    assert g0[0, 2] == pytest.approx(max(1e-16, abs(w0[0, 2]) ** 2), rel=1e-5)
+    zp = np.loadtxt(os.path.join(out, "MC_Results", "YGMXG_wZero_DFT_zzSpec" + tag0 + ".dat"))
+    assert zp.shape == (4 * l2 + 1, 2)
+    assert zp[l2, 1] == pytest.approx(max(1e-16, abs(w0[0, 2]) ** 2), rel=1e-11)

@@ -366,7 +366,8 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           double y = (double)Sy[p] + cfac * D[u][1];
           double z = (double)Sz[p] + cfac * D[u][2];
           const double nr = sqrt(x * x + y * y + z * z);
-          x /= nr; y /= nr; z /= nr;
+          const double inr = 1.0 / nr;
+          x *= inr; y *= inr; z *= inr;
           qs += spin0(u, p, 0) * x + spin0(u, p, 1) * y + spin0(u, p, 2) * z;
           const T tx = (T)x, ty = (T)y, tz = (T)z;
           Sx[p] = tx; Sy[p] = ty; Sz[p] = tz;
@@ -579,7 +580,8 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
         double y = (double)Sy[p] + cfac * D[Npad + p];
         double z = (double)Sz[p] + cfac * D[2 * Npad + p];
         const double nr = sqrt(x * x + y * y + z * z);
-        x /= nr; y /= nr; z /= nr;
+        const double inr = 1.0 / nr;
+        x *= inr; y *= inr; z *= inr;
         qs += (double)g0[p] * x + (double)g0[Npad + p] * y + (double)g0[2 * Npad + p] * z;
         const T tx = (T)x, ty = (T)y, tz = (T)z;
         Sx[p] = tx; Sy[p] = ty; Sz[p] = tz;

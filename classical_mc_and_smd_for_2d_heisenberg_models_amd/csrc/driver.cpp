@@ -153,7 +153,6 @@ Settings read_settings(const std::string &dir, bool strict) {
   bool okst = false;
   for (const char *x : states) okst |= (s.IState == x);
   if (!okst) fail(HSMD_ERR_GUARD, "Initial state string is invalid (check inputs);");
-  if (s.IState == "inputf") fail(HSMD_ERR_GUARD, "initial state 'inputf' (read from file) is not built");
   const std::string g2 = qc ? "square" : s.geom;
   if (g2 == "square" && s.IState == "2up1dw") fail(HSMD_ERR_GUARD, "Initial state string is not compatible with the chosen system geometry;");
   if (g2 != "square" && g2 != "triang" && s.IState != "high_T" && s.IState != "zpolar" && s.IState != "inputf")
@@ -304,6 +303,29 @@ void initial_state(const Settings &s, int L, int N, bool c3, int rank, uint64_t 
   };
   const double pi = acos(-1.0);
   for (int k = 0; k < 3 * N; k++) S[k] = 0.0;
+  if (s.IState == "inputf") {
+    // CF/mc_hm_code.cpp:1937-2025: tabulated graphs start from the reference
+    // stripe maps stateStrp{1..4} (x component, rank % 4); generated
+    // lattices from outBin/inputSConf.bin (N x 3 f64) for every rank
+    std::string f;
+    if (s.code[QCRYSTAL])
+      f = s.dir + "/outBin/stateStrp" + std::to_string(1 + rank % 4) + "_AppxtSz(" +
+          std::to_string(N) + ").bin";
+    else
+      f = s.dir + "/outBin/inputSConf.bin";
+    long long sz = 0;
+    if (!file_size(f, sz) || sz != (long long)N * 24) fail(HSMD_ERR_IO, "Error: invalid input spin file " + f);
+    const std::vector<double> v = read_bin<double>(f, (size_t)3 * N);
+    for (int k = 0; k < N; k++) {
+      if (s.code[QCRYSTAL]) S[3 * k] = v[3 * k];
+      else for (int d = 0; d < 3; d++) S[3 * k + d] = v[3 * k + d];
+    }
+    // the reference aborts on states whose average norm is not 1
+    double nrm = 0.0;
+    for (int k = 0; k < N; k++) nrm += sqrt(S[3 * k] * S[3 * k] + S[3 * k + 1] * S[3 * k + 1] + S[3 * k + 2] * S[3 * k + 2]);
+    if (fabs(nrm / N - 1.0) > 1.0e-6) fail(HSMD_ERR_IO, "Error: input spin state is not normalised: " + f);
+    return;
+  }
   if (s.code[ISI_MODEL]) {
     for (int k = 0; k < N; k++) {
       if (s.IState == "zpolar") S[3 * k] = 1.0;
@@ -481,11 +503,12 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
   sp.n_or = 5; sp.n_hb = 5; sp.n_or_boost = 2; sp.n_hb_boost = 8;
   sp.pt_cap50 = 1;
   sp.seed = seed;
+  // initial states first: host-only, so input errors surface before any GPU work
+  std::vector<double> spins((size_t)npt * N * 3);
+  for (int r = 0; r < npt; r++) initial_state(s, s.L, N, c3, r, seed, &spins[(size_t)r * N * 3]);
   hsmd_mc *mc = nullptr;
   check(hsmd_mc_create(lh.p, npt, T.data(), &sp, &mc));
   std::unique_ptr<hsmd_mc, void (*)(hsmd_mc *)> guard(mc, hsmd_mc_destroy);
-  std::vector<double> spins((size_t)npt * N * 3);
-  for (int r = 0; r < npt; r++) initial_state(s, s.L, N, c3, r, seed, &spins[(size_t)r * N * 3]);
   check(hsmd_mc_set_spins(mc, spins.data()));
   const bool pt = pc && s.code[PT_ON];
   // thermalisation: exchange from sweep 101 on, pair parity = sweep % 2

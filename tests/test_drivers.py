@@ -138,3 +138,34 @@ def test_mc_then_smd_pipeline_files_and_dssf(tmp_path):
     zp = np.loadtxt(os.path.join(out, "MC_Results", "YGMXG_wZero_DFT_zzSpec" + tag0 + ".dat"))
     assert zp.shape == (4 * l2 + 1, 2)
     assert zp[l2, 1] == pytest.approx(max(1e-16, abs(w0[0, 2]) ** 2), rel=1e-11)
+
+
+def test_inputf_state_errors_before_any_gpu_work(tmp_path):
+    # IState 'inputf' reads outBin/inputSConf.bin (N x 3 f64); a missing or
+    # unnormalised file stops the run (CF/mc_hm_code.cpp:1978-2047)
+    d = str(tmp_path)
+    default(d, IState="inputf")
+    with pytest.raises(H.HSMDError) as e:
+        H.run_mc(d, 4)
+    assert e.value.code == 4 and "inputSConf" in str(e.value)
+    np.full((36, 3), 0.5).tofile(os.path.join(d, "outBin", "inputSConf.bin"))
+    with pytest.raises(H.HSMDError) as e:
+        H.run_mc(d, 4)
+    assert e.value.code == 4 and "normalised" in str(e.value)
+
+
+@pytest.mark.gpu
+def test_inputf_ferromagnet_stays_ordered_at_low_temperature(tmp_path):
+    d = str(tmp_path)
+    default(d, IState="inputf", J1=-1.0, J2=0.0, T1=0.01, T2=0.02, NTerm=100, NMeas=2000,
+            crSkip=1000, rec=0, ptON=0)
+    s = np.zeros((36, 3))
+    s[:, 0] = 1.0
+    s.tofile(os.path.join(d, "outBin", "inputSConf.bin"))
+    H.run_mc(d, 2, precision=H.BATCHED_F64, seed=4)
+    T = np.fromfile(os.path.join(d, "outBin", "Temp_List_J2(0.000).bin"))
+    for t in T:
+        v1 = np.fromfile(os.path.join(d, "outBin", "MC_Bin_AvgValue1_T(%.5f)_J2(0.000).bin" % t))
+        v1 = v1.reshape(-1, 23)
+        assert np.all(v1[:, 15] > 0.98)      # |M|/N
+        assert np.all(v1[:, 0] < -1.95)      # E/N close to -2 J1 ... J1 = -1

@@ -115,6 +115,7 @@ SIGNATURES = {
     "hsmd_debug_philox": (_I, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, _I, _U32]),
     "hsmd_debug_exp": (_I, [_I, _D, _D]),
+    "hsmd_debug_mc_sweeps": (_I, [_P, _I]),
 }
 
 _lib = None
@@ -310,6 +311,11 @@ class MonteCarlo:
         _check(lib().hsmd_mc_run(self._h, int(n_sweeps), int(pt_parity), int(measure),
                                  None if tr is None else _dp(tr)))
         return tr
+
+    def sweeps_only(self, n_sweeps):
+        """n colour sweeps of every replica, no measurement or exchange
+        (times the sweep kernel alone)."""
+        _check(lib().hsmd_debug_mc_sweeps(self._h, int(n_sweeps)))
 
     def energies(self):
         e = np.zeros(self.R)

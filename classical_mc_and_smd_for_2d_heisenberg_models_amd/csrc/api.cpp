@@ -33,6 +33,7 @@ int poisson_disk(double box, double rmin, uint64_t seed, int max_n, double *out)
 Lattice *lattice_from_points(int n, const double *rvec, double box, const double r[3],
                              const hsmd_couplings &c);
 void mc_destroy(MCEngine *e);
+void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st);
 void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
                      double *energies);
 void debug_philox(uint64_t seed, uint32_t step, uint32_t inst, uint32_t stream, int n, uint32_t *out);
@@ -395,6 +396,15 @@ hsmd_status hsmd_debug_philox(uint64_t seed, uint32_t step, uint32_t inst, uint3
   return guarded([&] {
     REQUIRE(out && n >= 0, "bad argument");
     debug_philox(seed, step, inst, stream, n, out);
+  });
+}
+
+hsmd_status hsmd_debug_mc_sweeps(hsmd_mc *mc, int n) {
+  return guarded([&] {
+    REQUIRE(mc && n >= 0, "bad argument");
+    mc_launch_sweeps(*mc->e, n, 0);
+    const hipError_t er = hipDeviceSynchronize();
+    if (er != hipSuccess) throw Error(HSMD_ERR_DEVICE, std::string("sweeps: ") + hipGetErrorString(er));
   });
 }
 

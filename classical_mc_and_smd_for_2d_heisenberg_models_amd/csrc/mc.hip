@@ -65,7 +65,12 @@ struct SweepArgs {
   double *energy;          // per configuration
   double *obs;             // per configuration, NOBSX
   const double *stiff_w;   // 6 x Nb: J C0^2, J C1^2, J C2^2, J C0, J C1, J C2
+  const uint32_t *sweep_dev;   // graph replays: sweep index read here instead of `sweep`
 };
+
+__device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
+  return a.sweep_dev ? *a.sweep_dev : a.sweep;
+}
 
 // ------------------------------------------------------------ site updates
 
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a)
   const T *Jt = (const T *)a.Jt;
   for (int pass = 0; pass < nor + nhb; pass++) {
     const bool hb = pass >= nor;
-    const uint32_t step = a.sweep * 16u + (uint32_t)pass;
+    const uint32_t step = sweep_of(a) * 16u + (uint32_t)pass;
     for (int col = 0; col < a.n_colours; col++) {
       for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
         T hx, hy, hz;
@@ -389,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
       if (pass < nor) {
         overrelax<T>(hx, hy, hz, x, y, z);
       } else {
-        const uint32_t step = a.sweep * 16u + (uint32_t)pass;
+        const uint32_t step = sweep_of(a) * 16u + (uint32_t)pass;
         const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
         heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
       }
@@ -428,7 +433,7 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a
   const int slot = a.slot_of_config[c];
   const double beta = a.beta_slot[slot];
   const double *Jt = (const double *)a.Jt;
-  const uint32_t step = a.sweep * 16u;
+  const uint32_t step = sweep_of(a) * 16u;
   for (int col = 0; col < (a.metro_passes > 0 ? a.n_colours : 0); col++) {
     for (int p = a.col_off[col] + threadIdx.x; p < a.col_off[col + 1]; p += blockDim.x) {
       double h = a.cp.h_on ? -a.cp.h : 0.0;
@@ -512,6 +517,8 @@ struct PTArgs {
   double *moments;         // per slot, NMOM
   long long *n_meas;
   double *etrace;          // per slot energies of this sweep (nullable)
+  uint32_t *sweep_dev;     // graph replays: sweep index (advanced by this kernel)
+  int parity_alt;          // with sweep_dev: pair parity = sweep % 2
 };
 
 __device__ double stag_mag(const double *sub, int n0, double iN) {
@@ -529,6 +536,8 @@ __device__ double stag_mag(const double *sub, int n0, double iN) {
 
 __global__ void measure_exchange_kernel(const PTArgs a) {
   const int t = threadIdx.x;
+  const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
+  const int parity = (a.sweep_dev && a.parity_alt) ? (int)(sweep % 2u) : a.parity;
   const double iN = 1.0 / (double)a.N;
   if (t < a.R) {
     const int k = t;
@@ -578,8 +587,8 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
   __syncthreads();
   if (t < a.R) a.boost_slot[t] = 0;
   __syncthreads();
-  if (a.parity >= 0) {
-    const int k1 = a.parity + 2 * t;
+  if (parity >= 0) {
+    const int k1 = parity + 2 * t;
     if (k1 + 1 < a.R) {
       const int k2 = k1 + 1;
       const long long cnt1 = a.pt_cnt[k1] + 1, cnt2 = a.pt_cnt[k2] + 1;
@@ -590,7 +599,7 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
       const double dE = a.energy[c2] - a.energy[c1];
       const double dB = a.beta_slot[k2] - a.beta_slot[k1];
       const double fc = dB * dE;
-      const u32x4 r = draw4(a.seed, (uint32_t)k1, a.sweep, 0u, STREAM_PT);
+      const u32x4 r = draw4(a.seed, (uint32_t)k1, sweep, 0u, STREAM_PT);
       const double u = u01_f64(r.x, r.y);
       if (accept_exp(u, fc) && (!a.cap50 || ratio <= 50.0)) {
         a.config_of_slot[k1] = c2;
@@ -604,6 +613,18 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
       }
     }
   }
+  if (a.sweep_dev) {
+    __syncthreads();
+    if (t == 0) *a.sweep_dev = sweep + 1u;
+  }
+}
+
+// Sweep end in replayed graphs without an exchange: clears the boost flags
+// (set by the last exchange, used by the sweep just done) and advances the
+// device sweep counter.
+__global__ void sweep_bump_kernel(uint32_t *sweep_dev, int32_t *boost_slot, int R) {
+  for (int k = threadIdx.x; k < R; k += blockDim.x) boost_slot[k] = 0;
+  if (threadIdx.x == 0) *sweep_dev += 1u;
 }
 
 // ------------------------------------------------------------------ engine
@@ -638,6 +659,18 @@ struct MCEngine {
   DevBuf<double> sqfield;
   long long sq_count = 0;
   SweepArgs args{};
+  // captured sweep loop: one sweep (+ observables + exchange) as a graph,
+  // replayed per sweep with the sweep index kept on the device
+  DevBuf<uint32_t> sweep_dev;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gkey[3] = {-9, -9, -9};
+  ~MCEngine() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 static void fill_coup(const Lattice &L, Coup &cp, bool ising) {
@@ -934,6 +967,60 @@ static void launch_sweep(MCEngine &e, hipStream_t st) {
   HSMD_ZC_DISPATCH(e.lat.Z, launch_sweep_zc, e, st);
 }
 
+static void launch_obs(MCEngine &e, hipStream_t st) {
+  if (e.ising) hipLaunchKernelGGL(obs_kernel<int8_t>, dim3(e.R), dim3(kThreads), 0, st, e.args);
+  else if (e.prec == HSMD_BATCHED_F32)
+    hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, st, e.args);
+  else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, st, e.args);
+}
+
+// Sweep loop as a replayed hipGraph: the graph of one sweep (sweep kernel,
+// observables when measuring, exchange or a counter bump) is captured once
+// per (measure, parity mode, exchange) combination and launched n times; the
+// kernels read the sweep index (Philox counter, pair parity) from the device
+// counter the exchange kernel advances.  Same arithmetic and counters as the
+// launch-by-launch loop, which remains for energy traces, S(q) measurement
+// and the plain form.  No memset is recorded in the graph.
+static void run_graph(MCEngine &e, PTArgs p, int n, int pt_parity, int measure, bool post,
+                      int pthreads) {
+  if (!e.stream) HIP_CHECK(hipStreamCreate(&e.stream));
+  if (e.sweep_dev.n == 0) e.sweep_dev.alloc(1);
+  HIP_CHECK(hipMemcpy(e.sweep_dev.p, &e.sweep, sizeof(uint32_t), hipMemcpyHostToDevice));
+  const int key[3] = {measure, pt_parity, post ? 1 : 0};
+  if (!e.gexec || memcmp(key, e.gkey, sizeof key) != 0) {
+    if (e.gexec) { HIP_CHECK(hipGraphExecDestroy(e.gexec)); e.gexec = nullptr; }
+    if (e.graph) { HIP_CHECK(hipGraphDestroy(e.graph)); e.graph = nullptr; }
+    const SweepArgs saved = e.args;
+    e.args.sweep_dev = e.sweep_dev.p;
+    e.args.measure = measure;
+    HIP_CHECK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    launch_sweep(e, e.stream);
+    if (measure) launch_obs(e, e.stream);
+    if (post) {
+      p.sweep_dev = e.sweep_dev.p;
+      p.parity_alt = pt_parity == 2 ? 1 : 0;
+      p.parity = pt_parity == 2 ? 0 : pt_parity;
+      p.etrace = nullptr;
+      hipLaunchKernelGGL(measure_exchange_kernel, dim3(1), dim3(pthreads), 0, e.stream, p);
+    } else {
+      hipLaunchKernelGGL(sweep_bump_kernel, dim3(1), dim3(256), 0, e.stream, e.sweep_dev.p,
+                       e.boost.p, e.R);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e.stream, &g);
+    e.args = saved;
+    HIP_CHECK(ce);
+    e.graph = g;
+    HIP_CHECK(hipGraphInstantiate(&e.gexec, g, nullptr, nullptr, 0));
+    memcpy(e.gkey, key, sizeof key);
+  }
+  for (int s = 0; s < n; s++) HIP_CHECK(hipGraphLaunch(e.gexec, e.stream));
+  HIP_CHECK(hipStreamSynchronize(e.stream));
+  HIP_CHECK(hipGetLastError());
+  e.sweep += (uint32_t)n;
+  if (measure) e.meas_count += n;
+}
+
 void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrace_out) {
   if (!e.spins_set) throw Error(HSMD_ERR_STATE, "hsmd_mc_run before hsmd_mc_set_spins");
   if (n_sweeps < 0) throw Error(HSMD_ERR_ARG, "n_sweeps must be >= 0");
@@ -955,15 +1042,14 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
   const int pthreads = ((e.R + 63) / 64) * 64;
   // the measurement / exchange kernel is skipped when it has nothing to do
   const bool post = pt_parity >= 0 || measure || etrace_out;
+  if (e.prec != HSMD_PLAIN_F64 && !etrace_out && !(measure && e.sq_every > 0) && n_sweeps > 1) {
+    run_graph(e, p, n_sweeps, pt_parity, measure, post, pthreads);
+    return;
+  }
   for (int s = 0; s < n_sweeps; s++) {
     e.args.sweep = e.sweep;
     launch_sweep(e, 0);
-    if (measure) {
-      if (e.ising) hipLaunchKernelGGL(obs_kernel<int8_t>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
-      else if (e.prec == HSMD_BATCHED_F32)
-        hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
-      else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, 0, e.args);
-    }
+    if (measure) launch_obs(e, 0);
     if (measure && e.sq_every > 0 && e.meas_count % e.sq_every == 0) {
       const SpinType ty = e.ising ? SPIN_I8 : (e.prec == HSMD_BATCHED_F32 ? SPIN_F32 : SPIN_F64);
       sq_compute(*e.sq, e.spins.p, ty, e.R);

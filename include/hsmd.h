@@ -279,6 +279,10 @@ hsmd_status hsmd_run_smd(const char *workdir, int precision, int strict);
  * computed on the device (4n uint32), for the bit-exact stream check. */
 hsmd_status hsmd_debug_philox(uint64_t seed, uint32_t step, uint32_t inst,
                               uint32_t stream, int n, uint32_t *out);
+/* Launches n colour sweeps of every replica back to back (no measurement,
+ * no exchange) and waits for them: times the sweep kernel alone.  Advances
+ * the sweep counter like hsmd_mc_run. */
+hsmd_status hsmd_debug_mc_sweeps(hsmd_mc *mc, int n);
 /* Device det_exp_neg on n inputs, for the bit-exact acceptance check. */
 hsmd_status hsmd_debug_exp(int n, const double *x, double *y);
 

@@ -219,9 +219,9 @@ def cpu_baseline(lat, c, temps, seconds_target=15.0):
                       "for %d sweeps, one OpenMP thread per replica" % (R, n)}
 
 
-def read_traffic():
-    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary."""
-    p = os.path.join(ROOT, "profiles", "mc_sweep_f32_triang96_r256_pmc.txt")
+def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt"):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -355,11 +355,17 @@ def main():
         smd.step(k)     # k RK4 steps; every 1024 points the DSSF chunk GEMM runs
         sms = timer.stop()
         srate = sites * k / (sms * 1e-3)
-        # the chunk GEMM alone: fill a chunk to one short of full, then time the
-        # last step (one RK4 step + the flush) against a plain step
-        rem = 1024 - ((1024 + k) % 1024) - 1
-        if rem > 0:
-            smd.step(rem)
+        # complete the current trajectory chunk (its flush runs the GEMM)
+        fill = k % 1024
+        if fill:
+            smd.step(1024 - fill)
+        # the RK4 kernel alone: one launch advances every sample 1023 steps
+        # inside the next chunk (no flush)
+        timer.start()
+        smd.step(1023)
+        rk_ms = timer.stop() / 1023
+        # the chunk GEMM alone: the step that fills the chunk (one RK4 step +
+        # the flush) against a plain step
         timer.start()
         smd.step(1)
         t_flush = timer.stop()
@@ -369,17 +375,30 @@ def main():
         gemm_ms = max(t_flush - t_one, 1e-6)
         gm = sum(3 * lats[i].N for i in idx)
         mfma_flop = 2.0 * gm * 1024 * 1024 * 3      # M x (2 n_omega) x K, three split products
+        # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
+        # summed over sites: 6 (24 N + 32 n_bonds) per sample
+        rk_bytes = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
+        rk_gbs = rk_bytes / (rk_ms * 1e-3) / 1e9
         out["smd"] = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
                       "ms_per_step": sms / k, "samples": len(idx), "sites": sites,
                       "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
                                   "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
                                   "correction, DSSF time contraction over 512 bins included",
+                      "roofline": {"bound": "hbm", "achieved": rk_gbs, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": rk_gbs / HBM_PEAK_GBS,
+                                   "traffic": read_traffic("smd_rk4_f32_config4_pmc.txt"),
+                                   "kernel": "smd_kernel_two<float,4,8,12> (1023 steps per launch)",
+                                   "bytes_per_step": rk_bytes, "kernel_ms_per_step": rk_ms},
                       "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": 1024,
-                                          "mfma_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,
-                                          "peak_tflops_fp16_dense": 2500.0,
-                                          "mfma_util": mfma_flop / (gemm_ms * 1e-3) / 2.5e15}}
-        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s, chunk GEMM %.2f ms "
-              "(%.0f TFLOP/s)" % (sms / k, srate, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
+                                          "roofline": {"bound": "mfma",
+                                                       "achieved": mfma_flop / (gemm_ms * 1e-3) / 1e12,
+                                                       "peak": 2500.0, "unit": "TFLOP/s",
+                                                       "frac": mfma_flop / (gemm_ms * 1e-3) / 2.5e15,
+                                                       "kernel": "time_gemm_mfma_q_kernel",
+                                                       "flop_per_launch": mfma_flop}}}
+        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s, RK4 kernel %.4f ms/step "
+              "(%.0f GB/s gather model), chunk GEMM %.2f ms (%.0f TFLOP/s)"
+              % (sms / k, srate, rk_ms, rk_gbs, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
               file=sys.stderr, flush=True)
         del smd
         c2 = H.Couplings(J1=1.0, J2=0.6)

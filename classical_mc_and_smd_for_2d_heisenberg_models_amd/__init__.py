@@ -61,6 +61,23 @@ class SMDSpec(ctypes.Structure):
                 ("dssf", ctypes.c_int), ("gauss_seidel", ctypes.c_int)]
 
 
+class MCTuning(ctypes.Structure):
+    _fields_ = [("colour_cap", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("ms_per_sweep", ctypes.c_double), ("default_ms", ctypes.c_double),
+                ("n_candidates", ctypes.c_int32), ("cached", ctypes.c_int32)]
+
+
+class SMDTuning(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int32), ("gemm", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double), ("default_ms_per_step", ctypes.c_double),
+                ("gemm_ms", ctypes.c_double), ("default_gemm_ms", ctypes.c_double),
+                ("n_candidates", ctypes.c_int32), ("cached", ctypes.c_int32)]
+
+
+def _as_dict(st):
+    return {k: getattr(st, k) for k, _ in st._fields_}
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.POINTER(ctypes.c_double)
@@ -83,6 +100,12 @@ SIGNATURES = {
     "hsmd_lattice_grid_map": (_I, [_P, _I32]),
     "hsmd_lattice_bonds": (_I, [_P, _I32, _I32]),
     "hsmd_lattice_colours": (_I, [_P, _I32]),
+    "hsmd_lattice_set_colour_cap": (_I, [_P, ctypes.c_int32]),
+    "hsmd_mc_set_threads": (_I, [_P, _I]),
+    "hsmd_mc_autotune": (_I, [_P, _I, _D, ctypes.POINTER(MCSpec), _I, ctypes.POINTER(MCTuning)]),
+    "hsmd_smd_set_shape": (_I, [_P, _I, _I]),
+    "hsmd_smd_autotune": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D, _I,
+                               ctypes.POINTER(SMDTuning)]),
     "hsmd_lattice_destroy": (None, [_P]),
     "hsmd_local_fields": (_I, [_P, _I, _I, _D, _D]),
     "hsmd_energies": (_I, [_P, _I, _I, _D, _D]),
@@ -212,6 +235,12 @@ class Lattice:
         _check(lib().hsmd_lattice_set_impurities(self._h, _ip(imp)))
         self._refresh()
 
+    def set_colour_cap(self, cap):
+        """Colour schedule of the GPU sweeps: 0 = default, > 0 = first-fit
+        with at most cap sites per colour."""
+        _check(lib().hsmd_lattice_set_colour_cap(self._h, int(cap)))
+        self._refresh()
+
     def set_vacancies(self, vac):
         vac = np.ascontiguousarray(vac, dtype=np.int32)
         _check(lib().hsmd_lattice_set_vacancies(self._h, _ip(vac)))
@@ -280,17 +309,33 @@ class MonteCarlo:
     """All parallel-tempering replicas of one lattice on the GPU."""
 
     def __init__(self, lattice, temps, model=HEISENBERG, precision=BATCHED_F32, n_or=5, n_hb=5,
-                 n_or_boost=2, n_hb_boost=8, pt_cap50=True, seed=1):
+                 n_or_boost=2, n_hb_boost=8, pt_cap50=True, seed=1, tune=False, tune_sweeps=10):
+        """tune=True runs the sweep autotuner first (colour schedule and
+        workgroup shape, timed on the device, cached through HSMD_TUNE_CACHE)
+        and applies its choice: the lattice's colour cap changes."""
         self.lattice = lattice
         self.temps = np.ascontiguousarray(temps, dtype=np.float64)
         self.R = len(self.temps)
         self.N = lattice.N
         self.spec = MCSpec(model, precision, n_or, n_hb, n_or_boost, n_hb_boost,
                            int(pt_cap50), seed)
+        self.tuning = None
+        if tune:
+            t = MCTuning()
+            _check(lib().hsmd_mc_autotune(lattice._h, self.R, _dp(self.temps),
+                                          ctypes.byref(self.spec), int(tune_sweeps), ctypes.byref(t)))
+            self.tuning = _as_dict(t)
+            lattice.set_colour_cap(t.colour_cap)
         h = _P()
         _check(lib().hsmd_mc_create(lattice._h, self.R, _dp(self.temps), ctypes.byref(self.spec),
                                     ctypes.byref(h)))
         self._h = h
+        if self.tuning is not None and precision != PLAIN_F64:
+            self.set_threads(self.tuning["threads"])
+
+    def set_threads(self, threads):
+        """Threads per replica workgroup of the batched forms."""
+        _check(lib().hsmd_mc_set_threads(self._h, int(threads)))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -359,7 +404,10 @@ class SpinDynamics:
     """All SMD samples (any mix of lattices) on the GPU."""
 
     def __init__(self, lattices, sample_lattice, dt, n_steps, n_omega=None, precision=BATCHED_F32,
-                 dssf=True, window=None):
+                 dssf=True, window=None, tune=False, tune_steps=64):
+        """tune=True runs the step autotuner first (workgroup form and DSSF
+        contraction tile, timed on the device on a scratch engine, cached
+        through HSMD_TUNE_CACHE) and applies its choice."""
         self.lattices = list(lattices)
         self.sample_lattice = np.ascontiguousarray(sample_lattice, dtype=np.int32)
         self.S = len(self.sample_lattice)
@@ -368,18 +416,34 @@ class SpinDynamics:
         self.spec = SMDSpec(precision, dt, self.n_steps, self.n_omega, int(dssf), 0)
         self.window = None if window is None else np.ascontiguousarray(window, dtype=np.float64)
         arr = (_P * len(self.lattices))(*[l._h for l in self.lattices])
+        self.tuning = None
+        if tune:
+            t = SMDTuning()
+            _check(lib().hsmd_smd_autotune(arr, len(self.lattices), _ip(self.sample_lattice), self.S,
+                                           ctypes.byref(self.spec),
+                                           None if self.window is None else _dp(self.window),
+                                           int(tune_steps), ctypes.byref(t)))
+            self.tuning = _as_dict(t)
         h = _P()
         _check(lib().hsmd_smd_create(arr, len(self.lattices), _ip(self.sample_lattice), self.S,
                                      ctypes.byref(self.spec),
                                      None if self.window is None else _dp(self.window),
                                      ctypes.byref(h)))
         self._h = h
+        if self.tuning is not None:
+            self.set_shape(self.tuning["form"], self.tuning["gemm"] if dssf else 1)
         self.sizes = [self.lattices[i].N for i in self.sample_lattice]
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.hsmd_smd_destroy(self._h)
             self._h = None
+
+    def set_shape(self, form, gemm):
+        """Launch shape of the batched forms: form 1 (LDS, one slot class),
+        2 (LDS, per-workgroup classes), 3 (global memory); gemm 0 (128 x 128
+        contraction tile) or 1 (64 x 64)."""
+        _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
 
     def set_samples(self, spins_list):
         flat = np.concatenate([np.asarray(s, dtype=np.float64).reshape(-1) for s in spins_list])

@@ -34,6 +34,9 @@ Lattice *lattice_from_points(int n, const double *rvec, double box, const double
                              const hsmd_couplings &c);
 void mc_destroy(MCEngine *e);
 void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st);
+void mc_set_threads(MCEngine &e, int threads);
+void mc_autotune(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp, int n_sweeps,
+                 hsmd_mc_tuning *out);
 void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
                      double *energies);
 void debug_philox(uint64_t seed, uint32_t step, uint32_t inst, uint32_t stream, int n, uint32_t *out);
@@ -48,6 +51,9 @@ void smd_dssf(SMDEngine &e, int li, double *out, double *qss);
 void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out);
 void smd_energies(SMDEngine &e, double *E);
 void smd_destroy(SMDEngine *e);
+void smd_set_shape(SMDEngine &e, int form, int gemm);
+void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
+                  const hsmd_smd_spec &sp, const double *window, int n_steps, hsmd_smd_tuning *out);
 
 }  // namespace hsmd
 
@@ -139,6 +145,21 @@ hsmd_status hsmd_lattice_set_vacancies(hsmd_lattice *lat, const int32_t *vac) {
     for (int i = 0; i < lat->lat.N; i++) REQUIRE(vac[i] == 0 || vac[i] == 1, "vacancy map must be 0/1");
     lat->lat.vacant.assign(vac, vac + lat->lat.N);
     lat->lat.rebuild();
+  });
+}
+
+hsmd_status hsmd_lattice_set_colour_cap(hsmd_lattice *lat, int32_t cap) {
+  return guarded([&] {
+    REQUIRE(lat && cap >= 0, "bad argument");
+    const int old = lat->lat.col_cap;
+    lat->lat.col_cap = cap;
+    try {
+      lat->lat.rebuild();
+    } catch (...) {
+      lat->lat.col_cap = old;
+      lat->lat.rebuild();
+      throw;
+    }
   });
 }
 
@@ -249,6 +270,21 @@ hsmd_status hsmd_mc_set_spins(hsmd_mc *mc, const double *spins) {
   });
 }
 
+hsmd_status hsmd_mc_set_threads(hsmd_mc *mc, int threads) {
+  return guarded([&] {
+    REQUIRE(mc, "null argument");
+    mc_set_threads(*mc->e, threads);
+  });
+}
+
+hsmd_status hsmd_mc_autotune(const hsmd_lattice *lat, int R, const double *temps,
+                             const hsmd_mc_spec *spec, int n_sweeps, hsmd_mc_tuning *out) {
+  return guarded([&] {
+    REQUIRE(lat && temps && spec && out, "null argument");
+    mc_autotune(lat->lat, R, temps, *spec, n_sweeps, out);
+  });
+}
+
 hsmd_status hsmd_mc_get_spins(hsmd_mc *mc, double *spins) {
   return guarded([&] {
     REQUIRE(mc && spins, "null argument");
@@ -324,6 +360,27 @@ void hsmd_mc_destroy(hsmd_mc *mc) {
   if (!mc) return;
   mc_destroy(mc->e);
   delete mc;
+}
+
+hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm) {
+  return guarded([&] {
+    REQUIRE(smd, "null argument");
+    smd_set_shape(*smd->e, form, gemm);
+  });
+}
+
+hsmd_status hsmd_smd_autotune(const hsmd_lattice *const *lattices, int n_lattices,
+                              const int32_t *sample_lattice, int S, const hsmd_smd_spec *spec,
+                              const double *window, int n_steps, hsmd_smd_tuning *out) {
+  return guarded([&] {
+    REQUIRE(lattices && sample_lattice && spec && out && n_lattices > 0, "null argument");
+    std::vector<const Lattice *> ls(n_lattices);
+    for (int i = 0; i < n_lattices; i++) {
+      REQUIRE(lattices[i], "null lattice");
+      ls[i] = &lattices[i]->lat;
+    }
+    smd_autotune(ls.data(), n_lattices, sample_lattice, S, *spec, window, n_steps, out);
+  });
 }
 
 hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,

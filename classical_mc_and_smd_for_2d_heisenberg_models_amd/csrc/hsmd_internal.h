@@ -53,6 +53,9 @@ struct Lattice {
   bool j2_on = false, jx_on = false, xxz_on = false, h_on = false;
   bool disorder = false, j2ab_on = false, c3sym = false;
   Labels labels;
+  // colour schedule: 0 = default (first-fit, rebalanced when cheap);
+  // > 0 = first-fit with at most col_cap sites per colour (autotuner)
+  int col_cap = 0;
 
   // --- derived (rebuild())
   std::vector<int32_t> bi, bj, bshell;  // reference bond list (make_bondList order)
@@ -96,6 +99,14 @@ Lattice *lattice_generate(const std::string &geom, int L, bool pbc, bool strict,
 Lattice *lattice_from_tables(int n, int zn1, int zn2, int znx, const int32_t *nb,
                              const double *rvec, const hsmd_couplings &c);
 void derive_flags(Lattice &lat);
+
+// tune.cpp: signature hashing and the optional on-disk tuning cache
+// (path in the environment variable HSMD_TUNE_CACHE; no caching when unset)
+uint64_t fnv1a(const void *data, size_t n, uint64_t h = 0xcbf29ce484222325ull);
+uint64_t lattice_signature(const Lattice &L);
+bool tune_cache_get(const std::string &key, std::vector<double> &vals);
+void tune_cache_put(const std::string &key, const std::vector<double> &vals);
+std::string device_tag();
 
 }  // namespace hsmd
 

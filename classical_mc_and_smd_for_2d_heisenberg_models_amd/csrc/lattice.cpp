@@ -509,11 +509,16 @@ void Lattice::rebuild() {
       return (int)cnt.size();
     };
     std::vector<int32_t> col;
-    int nc = greedy(N, col);
-    int cap = (N + nc - 1) / nc;
-    std::vector<int32_t> col2;
-    int nc2 = greedy(cap, col2);
-    if (nc2 <= nc + 2) { col = col2; nc = nc2; }
+    int nc;
+    if (col_cap > 0) {
+      nc = greedy(col_cap, col);
+    } else {
+      nc = greedy(N, col);
+      int cap = (N + nc - 1) / nc;
+      std::vector<int32_t> col2;
+      int nc2 = greedy(cap, col2);
+      if (nc2 <= nc + 2) { col = col2; nc = nc2; }
+    }
     if (nc > CMAX) throw Error(HSMD_ERR_ARG, "graph needs too many colours");
     colour = col;
     n_colours = nc;

@@ -643,6 +643,11 @@ struct MCEngine {
   DevBuf<int32_t> nb, perm, bi, bj, z2nb;
   DevBuf<uint32_t> nb16;
   bool fast = false;
+  // launch shape of the batched sweeps: threads per replica workgroup; the
+  // register-label kernel needs every colour class to fit one pass of them
+  int threads = kThreads;
+  bool fast_ok = false;
+  int maxcol = 0;
   DevBuf<uint8_t> bc;
   DevBuf<int8_t> lab;
   DevBuf<float> Jf;
@@ -834,8 +839,10 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   {
     int maxcol = 0;
     for (int k = 0; k < lat.n_colours; k++) maxcol = std::max(maxcol, lat.col_off[k + 1] - lat.col_off[k]);
-    e->fast = e->use_lds && !e->ising && !lat.general && (lat.N + 1) < 65535 && maxcol <= kThreads;
-    if (e->fast) {
+    e->maxcol = maxcol;
+    e->fast_ok = e->use_lds && !e->ising && !lat.general && (lat.N + 1) < 65535 && maxcol <= kThreads;
+    e->fast = e->fast_ok;
+    if (e->fast_ok) {
       const int ZH = (lat.Z + 1) / 2;
       std::vector<uint32_t> pk((size_t)std::max(ZH, 1) * N);
       for (int k = 0; k < ZH; k++)
@@ -935,17 +942,17 @@ static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
   a.config_base = 0;
   const size_t sh = e.use_lds ? e.lds_bytes : 0;
   if (e.ising) {
-    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
   } else if (e.fast && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
-    if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
+    if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
   } else if (f32) {
-    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_kernel<float, false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<float, false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
   } else {
-    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<double, true, ZC>), dim3(e.R), dim3(kThreads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_kernel<double, false, ZC>), dim3(e.R), dim3(kThreads), 0, st, a);
+    if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<double, true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_kernel<double, false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
   }
 }
 
@@ -957,7 +964,7 @@ static void set_lds_attr(MCEngine &e) {
   else if (f32) fn = (const void *)heis_sweep_kernel<float, true, ZC>;
   else fn = (const void *)heis_sweep_kernel<double, true, ZC>;
   HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
-  if (e.fast) {
+  if (e.fast_ok) {
     fn = f32 ? (const void *)heis_sweep_fast<float, ZC> : (const void *)heis_sweep_fast<double, ZC>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
   }
@@ -1159,6 +1166,130 @@ void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st) {
     e.args.measure = 0;
     launch_sweep(e, st);
   }
+}
+
+// Threads per replica workgroup of the batched sweeps (a multiple of 64 in
+// [64, 1024]).  Per-site arithmetic does not depend on it; block reductions
+// (energies) group their terms by wave, so FP64 energies can differ in the
+// last bits between shapes.
+void mc_set_threads(MCEngine &e, int threads) {
+  if (threads < 64 || threads > kThreads || threads % 64 != 0)
+    throw Error(HSMD_ERR_ARG, "threads per workgroup must be a multiple of 64 in [64, 1024]");
+  e.threads = threads;
+  e.fast = e.fast_ok && e.maxcol <= threads;
+  if (e.gexec) { HIP_CHECK(hipGraphExecDestroy(e.gexec)); e.gexec = nullptr; }
+  if (e.graph) { HIP_CHECK(hipGraphDestroy(e.graph)); e.graph = nullptr; }
+  e.gkey[0] = e.gkey[1] = e.gkey[2] = -9;
+}
+
+// Autotuner of the batched sweep: colour schedule x workgroup shape.
+//
+// Colour schedules are first-fit colourings with a cap on the sites per
+// colour: the library default, no cap, and caps of k x (64, 256, 512, 1024)
+// sites; a pass over a colour class of n sites takes ceil(n / threads)
+// rounds separated by barriers, so the best cap depends on the workgroup
+// width and on how the graph packs.  Workgroup shapes are 256, 512 and 1024
+// threads (the plain form has one shape).  Every distinct (colouring, shape)
+// pair is timed on the device over n_sweeps sweeps of R replicas of random
+// spins after two warm-up sweeps; the fastest wins.  Results are keyed by
+// the lattice signature, replica count, model, precision, pass composition
+// and device, and kept in the tuning cache when one is configured.
+void mc_autotune(const Lattice &L0, int R, const double *temps, const hsmd_mc_spec &sp, int n_sweeps,
+                 hsmd_mc_tuning *out) {
+  if (n_sweeps < 1) throw Error(HSMD_ERR_ARG, "n_sweeps must be >= 1");
+  char key[256];
+  snprintf(key, sizeof key, "mc_%016llx_R%d_m%d_p%d_%d.%d.%d.%d_%s",
+           (unsigned long long)lattice_signature(L0), R, sp.model, sp.precision, sp.n_or, sp.n_hb,
+           sp.n_or_boost, sp.n_hb_boost, device_tag().c_str());
+  std::vector<double> cv;
+  if (tune_cache_get(key, cv) && cv.size() >= 4) {
+    out->colour_cap = (int32_t)cv[0];
+    out->threads = (int32_t)cv[1];
+    out->ms_per_sweep = cv[2];
+    out->default_ms = cv[3];
+    out->n_candidates = 0;
+    out->cached = 1;
+    return;
+  }
+  const int N = L0.N;
+  std::vector<int> caps = {0, N};
+  for (int w : {64, 256, 512, 1024})
+    for (int k = 1; k <= 16; k++)
+      if (k * w < N && k * w * kCMAX >= N) caps.push_back(k * w);
+  std::vector<int> shapes = {1024};
+  if (sp.precision != HSMD_PLAIN_F64) shapes = {1024, 512, 256};
+  // random unit (or +-1) spins
+  std::vector<double> spins((size_t)R * N * 3, 0.0);
+  {
+    uint64_t st = 0x9e3779b97f4a7c15ull ^ (uint64_t)N;
+    auto u01 = [&]() {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (size_t i = 0; i < (size_t)R * N; i++) {
+      if (sp.model == HSMD_ISING) {
+        spins[3 * i] = u01() < 0.5 ? -1.0 : 1.0;
+      } else {
+        const double ph = 2.0 * M_PI * u01(), ct = 1.0 - 2.0 * u01(), stt = sqrt(1.0 - ct * ct);
+        spins[3 * i] = stt * cos(ph);
+        spins[3 * i + 1] = stt * sin(ph);
+        spins[3 * i + 2] = ct;
+      }
+    }
+  }
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  std::vector<std::vector<int32_t>> seen;
+  double best = 1e300, def_ms = -1.0;
+  int best_cap = 0, best_thr = kThreads, ncand = 0;
+  try {
+    for (int cap : caps) {
+      Lattice Lc = L0;
+      Lc.col_cap = cap;
+      try {
+        Lc.rebuild();
+      } catch (const Error &) {
+        continue;   // the cap needs more colours than a sweep supports
+      }
+      if (Lc.n_colours > kCMAX) continue;
+      bool dup = false;
+      for (auto &c : seen) dup = dup || c == Lc.col_off;
+      if (dup && cap != 0) continue;
+      seen.push_back(Lc.col_off);
+      for (int thr : shapes) {
+        std::unique_ptr<MCEngine> e(mc_create(Lc, R, temps, sp));
+        if (sp.precision != HSMD_PLAIN_F64) mc_set_threads(*e, thr);
+        mc_set_spins(*e, spins.data());
+        mc_launch_sweeps(*e, 2, 0);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipEventRecord(ev0, 0));
+        mc_launch_sweeps(*e, n_sweeps, 0);
+        HIP_CHECK(hipEventRecord(ev1, 0));
+        HIP_CHECK(hipEventSynchronize(ev1));
+        HIP_CHECK(hipGetLastError());
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        const double per = (double)ms / n_sweeps;
+        ncand++;
+        if (cap == 0 && thr == kThreads) def_ms = per;
+        if (per < best) { best = per; best_cap = cap; best_thr = thr; }
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    throw;
+  }
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  out->colour_cap = best_cap;
+  out->threads = best_thr;
+  out->ms_per_sweep = best;
+  out->default_ms = def_ms;
+  out->n_candidates = ncand;
+  out->cached = 0;
+  tune_cache_put(key, {(double)best_cap, (double)best_thr, best, def_ms});
 }
 
 // ------------------------------------------------- fields / energies (ABI)

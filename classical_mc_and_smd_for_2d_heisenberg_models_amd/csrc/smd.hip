@@ -205,6 +205,12 @@ struct SMDEngine {
   int nop = 0;                     // frequency bins held internally (n_omega padded)
   int ldt = 0;
   bool set = false;
+  // launch shape (hsmd_smd_set_shape): LDS-resident form possible, one slot
+  // class for every workgroup, and the contraction tile (0: 128 x 128 with
+  // pre-split twiddles, 1: 64 x 64)
+  bool lds_ok = false;
+  bool one_class = false;
+  int gemm = 0;
 };
 
 template <typename T>
@@ -241,25 +247,63 @@ static void upload_twiddles(SMDEngine &e) {
   }
 }
 
-// The SMD kernels keep the reference's site order rather than the MC colour
-// order: with it the neighbour gathers of consecutive sites (i +- 1, i +- L,
-// ...) fall on consecutive LDS banks and cache lines, while the colour order
-// scatters them.  Returns L with its device tables re-expressed in that order
-// (identity permutation).
+// The SMD kernels do not use the MC colour order, which scatters the
+// neighbours of consecutive sites.  Regular lattices keep the reference's
+// site order: the neighbour gathers of consecutive sites (i +- 1, i +- L,
+// ...) fall on consecutive LDS banks and cache lines.  Tabulated graphs
+// (point sets, quasicrystal tables) come in whatever order their generator
+// produced; they are put in Morton (Z-curve) order of their positions, so
+// that neighbours in space are neighbours in memory.  Sums over sites and
+// bonds are order-independent up to rounding; per-site results are not
+// affected at all.  Returns L with its device tables re-expressed in the
+// chosen order (perm: new label -> site, inv: site -> new label).
+static uint32_t spread16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
 static Lattice natural_order(const Lattice &L) {
   Lattice o = L;
   const int N = L.N, Z = L.Z;
-  for (int s = 0; s < Z; s++)
+  std::vector<int> ord(N);
+  for (int i = 0; i < N; i++) ord[i] = i;
+  if (L.tabulated && L.rvec.size() == (size_t)2 * N) {
+    double lo[2] = {L.rvec[0], L.rvec[1]}, hi[2] = {L.rvec[0], L.rvec[1]};
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 2; d++) {
+        lo[d] = std::min(lo[d], L.rvec[2 * (size_t)i + d]);
+        hi[d] = std::max(hi[d], L.rvec[2 * (size_t)i + d]);
+      }
+    std::vector<uint32_t> key(N);
     for (int i = 0; i < N; i++) {
-      const int q = L.g_nb[(size_t)s * N + L.inv[i]];
-      o.g_nb[(size_t)s * N + i] = q < N ? L.perm[q] : N;
+      uint32_t c[2];
+      for (int d = 0; d < 2; d++) {
+        const double w = hi[d] > lo[d] ? (L.rvec[2 * (size_t)i + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+        c[d] = (uint32_t)std::min(65535.0, std::max(0.0, w * 65535.0));
+      }
+      key[i] = spread16(c[0]) | (spread16(c[1]) << 1);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  for (int p = 0; p < N; p++) { o.perm[p] = ord[p]; o.inv[ord[p]] = p; }
+  for (int s = 0; s < Z; s++)
+    for (int p = 0; p < N; p++) {
+      const int q = L.g_nb[(size_t)s * N + L.inv[ord[p]]];
+      o.g_nb[(size_t)s * N + p] = q < N ? o.inv[L.perm[q]] : N;
     }
   if (!L.g_J.empty())
     for (int s = 0; s < Z; s++)
-      for (int i = 0; i < N; i++) o.g_J[(size_t)s * N + i] = L.g_J[(size_t)s * N + L.inv[i]];
-  o.g_bi = L.bi;
-  o.g_bj = L.bj;
-  for (int i = 0; i < N; i++) { o.perm[i] = i; o.inv[i] = i; }
+      for (int p = 0; p < N; p++) o.g_J[(size_t)s * N + p] = L.g_J[(size_t)s * N + L.inv[ord[p]]];
+  o.g_bi.resize(L.bi.size());
+  o.g_bj.resize(L.bj.size());
+  for (size_t b = 0; b < L.bi.size(); b++) {
+    o.g_bi[b] = o.inv[L.bi[b]];
+    o.g_bj[b] = o.inv[L.bj[b]];
+  }
   o.g_perm = o.perm;
   return o;
 }
@@ -400,6 +444,8 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->lds_bytes = (size_t)3 * 3 * e->lds_npad * es;   // S, B1, B2
   e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
                e->spt * (int)es <= 16;
+  e->lds_ok = e->use_lds;
+  e->gemm = (e->f32 && e->ldt > 0) ? 0 : 1;
   // register-resident global form only for the plain form's small lattices
   e->wide = !e->use_lds && (sp.precision != HSMD_PLAIN_F64 || e->spt > (e->f32 ? 8 : 4));
   if (!e->use_lds) {
@@ -478,7 +524,7 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.f32 && e.ldt > 0)
+  if (e.f32 && e.ldt > 0 && e.gemm == 0)
     launch_time_gemm_mfma_wide(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw_hi.p,
                                e.tw_lo.p, e.ldt, e.chunk_base, e.chunk_fill, ldb,
                                (float *)e.ybuf.p, 0);
@@ -524,7 +570,7 @@ void smd_step(SMDEngine &e, int n) {
     a.ks = e.ks.p;
     a.dd = e.dd.p;
     const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.minZ, e.general, e.use_lds, e.wide,
-                          e.spec.precision == HSMD_PLAIN_F64, e.lds_bytes};
+                          e.spec.precision == HSMD_PLAIN_F64, e.lds_bytes, e.one_class};
     if (e.f32) smd_launch_f32(lc, a); else smd_launch_f64(lc, a);
     HIP_CHECK(hipGetLastError());
     e.t_next += np;
@@ -683,5 +729,165 @@ void smd_energies(SMDEngine &e, double *E) {
 }
 
 void smd_destroy(SMDEngine *e) { delete e; }
+
+// True when the FP32 LDS batch would run per-workgroup slot classes.
+static bool two_class(const SMDEngine &e) {
+  if (!e.f32 || !e.lds_ok || e.general || e.spt > 4) return false;
+  return (e.minZ <= 8 && e.maxZ > 8 && e.maxZ <= 12) || (e.minZ <= 4 && e.maxZ > 4 && e.maxZ <= 8);
+}
+
+static int current_form(const SMDEngine &e) {
+  if (e.use_lds) return (two_class(e) && !e.one_class) ? 2 : 1;
+  return e.wide ? 3 : 0;
+}
+
+// Launch shape of the batched forms.  form: 0 keeps the current one;
+// 1 = LDS-resident, every workgroup with the batch's largest slot class;
+// 2 = LDS-resident, each workgroup with its own lattice's slot class (FP32
+// batches mixing two classes); 3 = global-memory form (one workgroup per
+// sample, stages in HBM / Infinity Cache).  gemm: 0 = 128 x 128 contraction
+// tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64.
+// Per-site arithmetic is the same in every shape; FP64 reductions may group
+// terms differently.
+void smd_set_shape(SMDEngine &e, int form, int gemm) {
+  if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
+  if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
+  if (gemm < 0 || gemm > 1) throw Error(HSMD_ERR_ARG, "gemm tile must be 0 or 1");
+  if ((form == 1 || form == 2) && !e.lds_ok)
+    throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
+  if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
+  if (gemm == 0 && !(e.f32 && e.ldt > 0))
+    throw Error(HSMD_ERR_ARG, "the 128 x 128 tile needs FP32 and 2 n_omega a multiple of 256");
+  if (form == 1 || form == 2) {
+    e.use_lds = true;
+    e.wide = false;
+    e.one_class = form == 1;
+  } else if (form == 3) {
+    const size_t es = e.f32 ? sizeof(float) : sizeof(double);
+    const size_t so = e.spins.n / es;
+    if (e.stage.n == 0) { e.stage.alloc(2 * so * es); e.stage.zero(); }
+    if (e.ks.n == 0) e.ks.alloc(6 * so * es);
+    if (e.dd.n == 0) e.dd.alloc(3 * so);
+    e.use_lds = false;
+    e.wide = true;
+    e.one_class = false;
+  }
+  e.gemm = gemm;
+}
+
+// Autotuner of the batched SMD step: workgroup form (LDS-resident with one
+// or two slot classes, global-memory) and the tile of the DSSF contraction.
+// A scratch engine with the caller's lattices, batch and spec runs n_steps
+// RK4 steps of random unit spins per form (no chunk flush inside) and one
+// full-chunk contraction per tile, timed with HIP events; the fastest of
+// each wins.  Cached like the MC tuner.
+void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
+                  const hsmd_smd_spec &sp, const double *window, int n_steps, hsmd_smd_tuning *out) {
+  if (n_steps < 1) throw Error(HSMD_ERR_ARG, "n_steps must be >= 1");
+  if (sp.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (int i = 0; i < nl; i++) {
+    const uint64_t ls = lattice_signature(*lats[i]);
+    h = fnv1a(&ls, sizeof ls, h);
+  }
+  h = fnv1a(sample_lattice, (size_t)S * sizeof(int32_t), h);
+  char key[256];
+  snprintf(key, sizeof key, "smd_%016llx_S%d_p%d_T%d_w%d_d%d_%s", (unsigned long long)h, S, sp.precision,
+           sp.n_steps, sp.n_omega, sp.dssf, device_tag().c_str());
+  std::vector<double> cv;
+  if (tune_cache_get(key, cv) && cv.size() >= 6) {
+    out->form = (int32_t)cv[0];
+    out->gemm = (int32_t)cv[1];
+    out->ms_per_step = cv[2];
+    out->default_ms_per_step = cv[3];
+    out->gemm_ms = cv[4];
+    out->default_gemm_ms = cv[5];
+    out->n_candidates = 0;
+    out->cached = 1;
+    return;
+  }
+  std::unique_ptr<SMDEngine> e(smd_create(lats, nl, sample_lattice, S, sp, window));
+  std::vector<double> spins;
+  {
+    uint64_t st = 0x2545f4914f6cdd1dull;
+    auto u01 = [&]() {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    size_t tot = 0;
+    for (auto &d : e->hs) tot += (size_t)d.N;
+    spins.resize(3 * tot);
+    for (size_t i = 0; i < tot; i++) {
+      const double ph = 2.0 * M_PI * u01(), ct = 1.0 - 2.0 * u01(), stt = sqrt(1.0 - ct * ct);
+      spins[3 * i] = stt * cos(ph);
+      spins[3 * i + 1] = stt * sin(ph);
+      spins[3 * i + 2] = ct;
+    }
+  }
+  const int def_form = current_form(*e), def_gemm = e->gemm;
+  std::vector<int> forms;
+  if (e->lds_ok) forms.push_back(1);
+  if (two_class(*e)) forms.push_back(2);
+  forms.push_back(3);
+  const int k = std::max(1, std::min(n_steps, std::min(e->Tc, sp.n_steps) - 3));
+  hipEvent_t ev0, ev1;
+  HIP_CHECK(hipEventCreate(&ev0));
+  HIP_CHECK(hipEventCreate(&ev1));
+  auto timed = [&](auto &&fn) {
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipEventRecord(ev0, 0));
+    fn();
+    HIP_CHECK(hipEventRecord(ev1, 0));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    return (double)ms;
+  };
+  double best = 1e300, def_ms = -1.0, gbest = -1.0, gdef = -1.0;
+  int best_form = def_form, best_gemm = def_gemm, ncand = 0;
+  try {
+    for (int f : forms) {
+      smd_set_shape(*e, f, def_gemm);
+      smd_set_samples(*e, spins.data());
+      smd_step(*e, 1);
+      const double per = timed([&] { smd_step(*e, k); }) / k;
+      ncand++;
+      if (f == def_form) def_ms = per;
+      if (per < best) { best = per; best_form = f; }
+    }
+    if (sp.dssf && e->f32) {
+      std::vector<int> tiles = {1};
+      if (e->ldt > 0) tiles = {0, 1};
+      gbest = 1e300;
+      for (int g : tiles) {
+        smd_set_shape(*e, best_form, g);
+        e->chunk_base = 0;
+        e->chunk_fill = std::min(e->Tc, sp.n_steps);
+        flush_chunk(*e);   // warm-up
+        e->chunk_base = 0;
+        e->chunk_fill = std::min(e->Tc, sp.n_steps);
+        const double ms = timed([&] { flush_chunk(*e); });
+        ncand++;
+        if (g == def_gemm) gdef = ms;
+        if (ms < gbest) { gbest = ms; best_gemm = g; }
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    throw;
+  }
+  HIP_CHECK(hipEventDestroy(ev0));
+  HIP_CHECK(hipEventDestroy(ev1));
+  out->form = best_form;
+  out->gemm = best_gemm;
+  out->ms_per_step = best;
+  out->default_ms_per_step = def_ms;
+  out->gemm_ms = gbest;
+  out->default_gemm_ms = gdef;
+  out->n_candidates = ncand;
+  out->cached = 0;
+  tune_cache_put(key, {(double)best_form, (double)best_gemm, best, def_ms, gbest, gdef});
+}
 
 }  // namespace hsmd

@@ -631,6 +631,7 @@ struct SmdLaunch {
   bool wide;      // global-memory wide form
   bool plain;     // one sample per launch (the reference's one instance per rank)
   size_t lds_bytes;
+  bool one_class; // LDS form: every workgroup uses the largest slot class
 };
 
 template <typename T, int ZC, bool GEN>
@@ -696,12 +697,12 @@ void smd_launch_zc(const SmdLaunch &c, const StepArgs &a) {
   const int Z = c.maxZ;
   // two slot classes in one LDS-resident batch (FP32 only)
   if constexpr (sizeof(T) == 4 && SPT <= 4) {
-    if (c.use_lds && !c.general && c.minZ <= 8 && Z > 8 && Z <= 12) {
+    if (c.use_lds && !c.one_class && !c.general && c.minZ <= 8 && Z > 8 && Z <= 12) {
       if (c.minZ <= 4) smd_launch_two<T, SPT, 4, 12>(c, a);
       else smd_launch_two<T, SPT, 8, 12>(c, a);
       return;
     }
-    if (c.use_lds && !c.general && c.minZ <= 4 && Z > 4 && Z <= 8) {
+    if (c.use_lds && !c.one_class && !c.general && c.minZ <= 4 && Z > 4 && Z <= 8) {
       smd_launch_two<T, SPT, 4, 8>(c, a);
       return;
     }

@@ -124,6 +124,13 @@ hsmd_status hsmd_lattice_grid_map(const hsmd_lattice *lat, int32_t *grid_map);
 hsmd_status hsmd_lattice_bonds(const hsmd_lattice *lat, int32_t *pairs, int32_t *shell);
 /* colour of each site (0..n_colours-1) used by the GPU sweeps. */
 hsmd_status hsmd_lattice_colours(const hsmd_lattice *lat, int32_t *colour);
+/* Colour schedule of the GPU sweeps (no counterpart in the reference, which
+ * visits sites in a shuffled order, get_shuffledList CF/subrouts.cpp:2666):
+ * cap = 0 restores the default (first-fit in site order, rebalanced to
+ * ceil(n / n_colours) sites per colour when that costs at most two more
+ * colours); cap > 0 is first-fit with at most cap sites per colour.  Engines
+ * created afterwards use it; hsmd_mc_autotune picks the cap. */
+hsmd_status hsmd_lattice_set_colour_cap(hsmd_lattice *lat, int32_t cap);
 void hsmd_lattice_destroy(hsmd_lattice *lat);
 
 /* --------------------------------------------------- fields and energies */
@@ -150,12 +157,36 @@ typedef struct hsmd_mc_spec {
   uint64_t seed;
 } hsmd_mc_spec;
 
+/* Result of hsmd_mc_autotune. */
+typedef struct hsmd_mc_tuning {
+  int32_t colour_cap;   /* for hsmd_lattice_set_colour_cap (0 = default schedule) */
+  int32_t threads;      /* for hsmd_mc_set_threads */
+  double ms_per_sweep;  /* measured with the chosen schedule and shape */
+  double default_ms;    /* measured with the default schedule and 1024 threads */
+  int32_t n_candidates; /* (schedule, shape) pairs timed; 0 when cached */
+  int32_t cached;       /* 1: read from the tuning cache (HSMD_TUNE_CACHE) */
+} hsmd_mc_tuning;
+
 /* R replicas, temps[R] per temperature slot (the reference's rank order). */
 hsmd_status hsmd_mc_create(const hsmd_lattice *lat, int R, const double *temps,
                            const hsmd_mc_spec *spec, hsmd_mc **out);
 /* Spins of the configuration at each temperature slot (R x n x 3 f64).
  * Ising spins are (+-1, 0, 0) as in the reference (CF/subrouts.cpp:4695-4725). */
 hsmd_status hsmd_mc_set_spins(hsmd_mc *mc, const double *spins);
+/* Threads per replica workgroup of the batched forms (multiple of 64 in
+ * [64, 1024]; default 1024).  No counterpart in the reference (one MPI rank
+ * per replica). */
+hsmd_status hsmd_mc_set_threads(hsmd_mc *mc, int threads);
+/* Autotuner of the batched sweep (no counterpart in the reference): times
+ * n_sweeps sweeps of R replicas on lat for every distinct colour schedule
+ * (first-fit colourings with caps of k x 64..1024 sites per colour, the
+ * default, no cap) and workgroup shape (256, 512, 1024 threads) on the
+ * device, and returns the fastest.  lat is not modified; apply the result
+ * with hsmd_lattice_set_colour_cap and hsmd_mc_set_threads.  With
+ * HSMD_TUNE_CACHE naming a file, results are cached there per lattice
+ * signature, R, model, precision, passes and device. */
+hsmd_status hsmd_mc_autotune(const hsmd_lattice *lat, int R, const double *temps,
+                             const hsmd_mc_spec *spec, int n_sweeps, hsmd_mc_tuning *out);
 hsmd_status hsmd_mc_get_spins(hsmd_mc *mc, double *spins);
 
 /* Runs n_sweeps sweeps of every replica; replaces the sweep loops
@@ -229,6 +260,34 @@ hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
                             const int32_t *sample_lattice, int S,
                             const hsmd_smd_spec *spec, const double *window,
                             hsmd_smd **out);
+/* Result of hsmd_smd_autotune. */
+typedef struct hsmd_smd_tuning {
+  int32_t form;              /* for hsmd_smd_set_shape: 1, 2 or 3 */
+  int32_t gemm;              /* for hsmd_smd_set_shape: 0 or 1 */
+  double ms_per_step;        /* RK4 step of the whole batch, chosen form */
+  double default_ms_per_step;/* same, form chosen at creation */
+  double gemm_ms;            /* one trajectory-chunk contraction, chosen tile (-1: no DSSF) */
+  double default_gemm_ms;
+  int32_t n_candidates;      /* shapes timed; 0 when cached */
+  int32_t cached;            /* 1: read from the tuning cache (HSMD_TUNE_CACHE) */
+} hsmd_smd_tuning;
+
+/* Launch shape of the batched forms (no counterpart in the reference, which
+ * runs one sample per rank).  form: 0 = keep, 1 = LDS-resident with the
+ * batch's largest slot class, 2 = LDS-resident with per-workgroup slot
+ * classes (FP32 batches of two classes), 3 = global-memory form.  gemm:
+ * 0 = 128 x 128 DSSF contraction tile (FP32, 2 n_omega a multiple of 256),
+ * 1 = 64 x 64.  Errors when the shape does not apply to the batch. */
+hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm);
+/* Autotuner of the batched SMD step (no counterpart in the reference): on a
+ * scratch engine of the same batch, times n_steps RK4 steps per applicable
+ * form and one trajectory-chunk contraction per tile, and returns the
+ * fastest of each; apply with hsmd_smd_set_shape.  Cached like
+ * hsmd_mc_autotune. */
+hsmd_status hsmd_smd_autotune(const hsmd_lattice *const *lattices, int n_lattices,
+                              const int32_t *sample_lattice, int S,
+                              const hsmd_smd_spec *spec, const double *window,
+                              int n_steps, hsmd_smd_tuning *out);
 /* Spins of every sample, concatenated in sample order (sum_s n_s x 3 f64). */
 hsmd_status hsmd_smd_set_samples(hsmd_smd *smd, const double *spins);
 hsmd_status hsmd_smd_get_samples(hsmd_smd *smd, double *spins);

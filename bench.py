@@ -42,6 +42,9 @@ sys.path.insert(0, ROOT)
 import classical_mc_and_smd_for_2d_heisenberg_models_amd as H  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
+SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
+                   3: "smd_kernel_wide<float,12,false>"}
 
 
 class HipTimer:
@@ -148,14 +151,15 @@ def q_path(n=200, qmax=2 * math.pi):
     return out
 
 
-def bench_graph(timer, rng, mc_steps=50, smd_steps=1024):
+def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=1024):
     c, lat, box = graph_workload(rng)
     N = lat.N
     print("[bench] graph: N=%d box=%.1f Z=%d colours=%d" % (N, box, lat.zn1 + lat.zn2 + lat.znx,
                                                             lat.n_colours), file=sys.stderr, flush=True)
     temps = np.geomspace(0.05, 1.5, 128)
     mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, n_or_boost=5,
-                      n_hb_boost=1, pt_cap50=False, seed=5)
+                      n_hb_boost=1, pt_cap50=False, seed=5, tune=tune, tune_sweeps=5)
+    mtune = mc.tuning
     mc.set_spins(unit_spins(rng, (128, N, 3)))
     mc.run(5, pt_parity=2)
     timer.start()
@@ -165,7 +169,8 @@ def bench_graph(timer, rng, mc_steps=50, smd_steps=1024):
     del mc
     S, T = 256, 4096
     smd = H.SpinDynamics([lat], np.zeros(S, np.int32), 0.01, T, n_omega=512,
-                         precision=H.BATCHED_F32)
+                         precision=H.BATCHED_F32, tune=tune, tune_steps=64)
+    stune = smd.tuning
     smd.set_samples([unit_spins(rng, (N, 3)) for _ in range(S)])
     smd.step(1023)
     timer.start()
@@ -185,9 +190,9 @@ def bench_graph(timer, rng, mc_steps=50, smd_steps=1024):
                         "J2=0.4 r<1.45, JX=0.1 r<1.75, 5%% vacancies" % (N, box),
             "sites": N, "colours": lat.n_colours,
             "mc": {"value": mrate, "unit": "spin-updates/s", "replicas": 128,
-                   "ms_per_sweep": mms / mc_steps, "sweeps": mc_steps},
+                   "ms_per_sweep": mms / mc_steps, "sweeps": mc_steps, "tuning": mtune},
             "smd": {"value": srate, "unit": "site-RK4-steps/s", "samples": S, "steps": smd_steps,
-                    "ms_per_step": sms / smd_steps, "n_steps_total": T},
+                    "ms_per_step": sms / smd_steps, "n_steps_total": T, "tuning": stune},
             "dssf_qpath": {"q_points": len(q), "ms": qms,
                            "tflops": dflop / (qms * 1e-3) / 1e12,
                            "finite": bool(np.isfinite(d).all())}}
@@ -265,6 +270,8 @@ def main():
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners (default launch shapes)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -285,8 +292,9 @@ def main():
     ups_per_sweep = R * N * 6
     print("[bench] MC workload: triang L=96 N=%d R=%d colours=%d" % (N, R, lat.n_colours),
           file=sys.stderr, flush=True)
+    tune = not args.no_tune
     mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, n_or_boost=5,
-                      n_hb_boost=1, pt_cap50=False, seed=3)
+                      n_hb_boost=1, pt_cap50=False, seed=3, tune=tune, tune_sweeps=5)
     mc.set_spins(unit_spins(rng, (R, N, 3)))
     mc.run(args.warmup, pt_parity=2)
     timer.start()
@@ -298,6 +306,9 @@ def main():
     timer.start()
     mc.sweeps_only(ksteps)
     kms = timer.stop() / ksteps
+    thr = mc.tuning["threads"] if mc.tuning else 1024
+    mc_kernel = ("heis_sweep_fast<float,20>" if np.bincount(lat.colours()).max() <= thr
+                 else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
     z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
     bytes_per_update = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
     achieved = ups_per_sweep * bytes_per_update / (kms * 1e-3) / 1e9
@@ -322,10 +333,11 @@ def main():
                                "JX=0.05 h=0.3 lambda=0.9, 256 replicas T in [0.02,2] geometric, "
                                "sweep = 5 OR + 1 HB + replica exchange",
                    "replicas": R, "sites": N, "colours": lat.n_colours,
-                   "pt_accept_rate": float(acc.sum() / max(1, att.sum()))},
+                   "pt_accept_rate": float(acc.sum() / max(1, att.sum())),
+                   "tuning": mc.tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "heis_sweep_fast<float,20>",
+                     "kernel": mc_kernel,
                      "bytes_per_update": bytes_per_update, "kernel_ms": kms},
     }
     del mc
@@ -347,7 +359,8 @@ def main():
         lats, idx, samples = smd_workload(rng)
         sites = sum(lats[i].N for i in idx)
         T = 8192
-        smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32)
+        smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
+                             tune=tune, tune_steps=64)
         smd.set_samples(samples)
         smd.step(1023)  # warm-up: time points 0..1023, i.e. one full trajectory chunk
         k = args.smd_steps
@@ -384,10 +397,13 @@ def main():
                       "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
                                   "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
                                   "correction, DSSF time contraction over 512 bins included",
+                      "tuning": smd.tuning,
                       "roofline": {"bound": "hbm", "achieved": rk_gbs, "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": rk_gbs / HBM_PEAK_GBS,
                                    "traffic": read_traffic("smd_rk4_f32_config4_pmc.txt"),
-                                   "kernel": "smd_kernel_two<float,4,8,12> (1023 steps per launch)",
+                                   "kernel": SMD_FORM_KERNEL.get(
+                                       smd.tuning["form"] if smd.tuning else 2, "?")
+                                   + " (1023 steps per launch)",
                                    "bytes_per_step": rk_bytes, "kernel_ms_per_step": rk_ms},
                       "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": 1024,
                                           "roofline": {"bound": "mfma",
@@ -427,7 +443,7 @@ def main():
         print("[bench] SMD square24: plain f64 %.3e, batched f32 %.3e" % (prate, brate),
               file=sys.stderr, flush=True)
     if not args.no_graph:
-        out["graph"] = bench_graph(timer, rng)
+        out["graph"] = bench_graph(timer, rng, tune)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, c, temps)
     print(json.dumps(out))

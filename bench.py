@@ -224,13 +224,13 @@ def cpu_baseline(lat, c, temps, seconds_target=15.0):
                       "for %d sweeps, one OpenMP thread per replica" % (R, n)}
 
 
-def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt"):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
+def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt", key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per step) from a committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(key):
             try:
                 return float(line.split("=")[1])
             except ValueError:
@@ -388,6 +388,8 @@ def main():
         gemm_ms = max(t_flush - t_one, 1e-6)
         gm = sum(3 * lats[i].N for i in idx)
         mfma_flop = 2.0 * gm * 1024 * 1024 * 3      # M x (2 n_omega) x K, three split products
+        # algorithmic DSSF contraction FLOP per chunk: 8 x 3 x Nq x n_omega x T per sample
+        alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * 1024 for i in idx)
         # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
         # summed over sites: 6 (24 N + 32 n_bonds) per sample
         rk_bytes = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
@@ -400,18 +402,25 @@ def main():
                       "tuning": smd.tuning,
                       "roofline": {"bound": "hbm", "achieved": rk_gbs, "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": rk_gbs / HBM_PEAK_GBS,
-                                   "traffic": read_traffic("smd_rk4_f32_config4_pmc.txt"),
+                                   "traffic": read_traffic("smd_rk4_f32_config4_pmc.txt",
+                                                           "traffic_bytes_per_step"),
+                                   "traffic_unit": "HBM bytes per RK4 step of the batch",
                                    "kernel": SMD_FORM_KERNEL.get(
                                        smd.tuning["form"] if smd.tuning else 2, "?")
                                    + " (1023 steps per launch)",
                                    "bytes_per_step": rk_bytes, "kernel_ms_per_step": rk_ms},
                       "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": 1024,
                                           "roofline": {"bound": "mfma",
-                                                       "achieved": mfma_flop / (gemm_ms * 1e-3) / 1e12,
+                                                       "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
                                                        "peak": 2500.0, "unit": "TFLOP/s",
-                                                       "frac": mfma_flop / (gemm_ms * 1e-3) / 2.5e15,
+                                                       "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
+                                                       "traffic": read_traffic(
+                                                           "dssf_contraction_mfma_pmc.txt"),
                                                        "kernel": "time_gemm_mfma_q_kernel",
-                                                       "flop_per_launch": mfma_flop}}}
+                                                       "algorithmic_flop_per_launch": alg_flop,
+                                                       "executed_flop_per_launch": mfma_flop,
+                                                       "executed_tflops":
+                                                           mfma_flop / (gemm_ms * 1e-3) / 1e12}}}
         print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s, RK4 kernel %.4f ms/step "
               "(%.0f GB/s gather model), chunk GEMM %.2f ms (%.0f TFLOP/s)"
               % (sms / k, srate, rk_ms, rk_gbs, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),

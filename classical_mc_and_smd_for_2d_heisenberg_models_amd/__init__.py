@@ -442,7 +442,7 @@ class SpinDynamics:
     def set_shape(self, form, gemm):
         """Launch shape of the batched forms: form 1 (LDS, one slot class),
         2 (LDS, per-workgroup classes), 3 (global memory); gemm 0 (128 x 128
-        contraction tile) or 1 (64 x 64)."""
+        contraction tile), 1 (64 x 64) or 2 (128 x 256)."""
         _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
 
     def set_samples(self, spins_list):

@@ -142,48 +142,75 @@ constexpr int WK = 32, WKP = WK + 8;
 // trajectory tile in that XCD's L2 (row-block count padded to a multiple of
 // 8).  Measured on MI355X: 47 ms per 1024-point chunk of the 512-sample
 // batch, 652 TFLOP/s counting the three split products.
-constexpr int QM = 128, QJ = 128, QT = 256;
+constexpr int QM = 128, QT = 256;
 
-__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(3, 3)))
-void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict__ A,
-                             const _Float16 *__restrict__ Bth, const _Float16 *__restrict__ Btl,
-                             int ldt, int k0, int K, int ldb, float *Y) {
-  __shared__ __align__(16) _Float16 Ah[QM * WKP], Al[QM * WKP];
-  __shared__ __align__(16) _Float16 Bh[QJ * WKP], Bl[QJ * WKP];
+// TJ = 128 (the main tile) or 256 (a wider tile: each trajectory value is
+// split and staged half as often and the chunk is read from memory half as
+// often, at 128 accumulator registers per lane and two workgroups per CU).
+template <int TJ>
+__device__ __forceinline__ void time_gemm_q_body(const GemmSample *samples, const float *__restrict__ A,
+                                                 const _Float16 *__restrict__ Bth,
+                                                 const _Float16 *__restrict__ Btl, int ldt, int k0,
+                                                 int K, int ldb, float *Y, _Float16 *Ah, _Float16 *Al,
+                                                 _Float16 *Bh, _Float16 *Bl) {
+  constexpr int NB = TJ / 32;               // 16-column blocks per wave
+  constexpr int BK = WK * TJ / QT;          // time points each thread stages for B
   const GemmSample sd = samples[blockIdx.z];
   const int M = sd.M, lda = sd.lda;
-  const int nj = ldb / QJ;
+  const int nj = ldb / TJ;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * QJ;
+  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * TJ;
   if (m0 >= M) return;
   const float *Ab = A + sd.a_off;
   float *Yb = Y + sd.y_off;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int wm = (wave >> 1) * 64, wj = (wave & 1) * 64;
+  const int wm = (wave >> 1) * 64, wj = (wave & 1) * (TJ / 2);
   const int li = lane & 15, lk = (lane >> 4) * 8;
   // staging: thread t owns row am = t % 128 (time points 16 kh .. + 15) and
-  // column bc = t % 128 (the same time points); 16 lanes write 16 rows
-  const int am = tid % QM, kh = tid / QM;
+  // column bc = t % TJ (BK time points from kb); 16 lanes write 16 rows
+  const int am = tid % QM;
+  const int kh = __builtin_amdgcn_readfirstlane(tid / QM);   // uniform per wave
   const bool arow = m0 + am < M;
-  const int bc = tid % QJ;
-  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + 16 * kh);
-  float4_t acc[4][4];
+  // trajectory loads of full time steps go through a buffer resource: the
+  // column (this thread's row of Y, clamped into the row stride; rows past M
+  // are computed and never stored) is the per-lane offset, the time row the
+  // per-wave scalar offset, so the sixteen loads need no address arithmetic
+  // and no bounds branches
+  const __amdgpu_buffer_rsrc_t arsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)Ab, (short)0, 0x7fffffff, 0x00020000);
+  const int acol = 4 * min(m0 + am, lda - 1);
+  const int lda4 = 4 * lda;
+  const int bc = tid % TJ, kb = (tid / TJ) * BK;
+  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + kb);
+  float4_t acc[4][NB];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NB; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  static_assert(BK == 16 || BK == 32, "B staging covers 16 or 32 time points per thread");
   float ra[16];
-  uint4 rbh0, rbh1, rbl0, rbl1;
+  uint4 rbh0, rbh1, rbl0, rbl1, rbh2, rbh3, rbl2, rbl3;
   auto fetch = [&](int kk) {
+    if (kk + WK <= K) {
+      const int r0 = (kk + 16 * kh) * lda4;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-      const int ka = kk + 16 * kh + e;
-      ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
+      for (int e = 0; e < 16; e++)
+        ra[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(arsrc, acol, r0 + e * lda4, 0));
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int ka = kk + 16 * kh + e;
+        ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
+      }
     }
     const uint4 *h = (const uint4 *)(Bth + bsrc + kk);
     const uint4 *l = (const uint4 *)(Btl + bsrc + kk);
     rbh0 = h[0]; rbh1 = h[1];
     rbl0 = l[0]; rbl1 = l[1];
+    if constexpr (BK == 32) {
+      rbh2 = h[2]; rbh3 = h[3];
+      rbl2 = l[2]; rbl3 = l[3];
+    }
   };
   auto stage = [&]() {
 #pragma unroll
@@ -191,17 +218,24 @@ void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict_
       half8_t hv, lv;
 #pragma unroll
       for (int e = 0; e < 8; e++) {
-        const _Float16 h = (_Float16)ra[8 * g + e];
-        hv[e] = h;
-        lv[e] = (_Float16)(ra[8 * g + e] - (float)h);
+        // hi: the value cut to FP16's 11 significant bits (exact in FP16 for
+        // |x| >= 2^-14); lo: the exact FP32 remainder, rounded to FP16
+        const float x = ra[8 * g + e];
+        const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) & 0xffffe000u);
+        hv[e] = (_Float16)h;
+        lv[e] = (_Float16)(x - h);
       }
       *(half8_t *)&Ah[am * WKP + 16 * kh + 8 * g] = hv;
       *(half8_t *)&Al[am * WKP + 16 * kh + 8 * g] = lv;
     }
-    uint4 *dh = (uint4 *)&Bh[bc * WKP + 16 * kh];
-    uint4 *dl = (uint4 *)&Bl[bc * WKP + 16 * kh];
+    uint4 *dh = (uint4 *)&Bh[bc * WKP + kb];
+    uint4 *dl = (uint4 *)&Bl[bc * WKP + kb];
     dh[0] = rbh0; dh[1] = rbh1;
     dl[0] = rbl0; dl[1] = rbl1;
+    if constexpr (BK == 32) {
+      dh[2] = rbh2; dh[3] = rbh3;
+      dl[2] = rbl2; dl[3] = rbl3;
+    }
   };
   fetch(0);
   for (int kk = 0; kk < K; kk += WK) {
@@ -216,7 +250,7 @@ void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict_
       al[a] = *(const half8_t *)&Al[row * WKP + lk];
     }
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < NB; b++) {
       const int col = wj + b * 16 + li;
       const half8_t bh = *(const half8_t *)&Bh[col * WKP + lk];
       const half8_t bl = *(const half8_t *)&Bl[col * WKP + lk];
@@ -232,7 +266,7 @@ void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict_
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < NB; b++) {
       const int j = j0 + wj + b * 16 + li;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -242,14 +276,38 @@ void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict_
     }
 }
 
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void time_gemm_mfma_q_kernel(const GemmSample *samples, const float *__restrict__ A,
+                             const _Float16 *__restrict__ Bth, const _Float16 *__restrict__ Btl,
+                             int ldt, int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[QM * WKP], Al[QM * WKP];
+  __shared__ __align__(16) _Float16 Bh[128 * WKP], Bl[128 * WKP];
+  time_gemm_q_body<128>(samples, A, Bth, Btl, ldt, k0, K, ldb, Y, Ah, Al, Bh, Bl);
+}
+
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void time_gemm_mfma_q256_kernel(const GemmSample *samples, const float *__restrict__ A,
+                                const _Float16 *__restrict__ Bth, const _Float16 *__restrict__ Btl,
+                                int ldt, int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[QM * WKP], Al[QM * WKP];
+  __shared__ __align__(16) _Float16 Bh[256 * WKP], Bl[256 * WKP];
+  time_gemm_q_body<256>(samples, A, Bth, Btl, ldt, k0, K, ldb, Y, Ah, Al, Bh, Bl);
+}
+
+// tj: 128 or 256 columns per workgroup tile
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
-                                int ldb, float *Y, hipStream_t st) {
-  if (ldb % QJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 128");
+                                int ldb, float *Y, hipStream_t st, int tj) {
+  if (tj != 128 && tj != 256) throw Error(HSMD_ERR_ARG, "contraction tile must be 128 or 256 columns");
+  if (ldb % tj != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of the tile width");
   const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
-  dim3 grid(nm * (ldb / QJ), 1, S);
-  hipLaunchKernelGGL(time_gemm_mfma_q_kernel, grid, dim3(QT), 0, st,
-                     (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
+  dim3 grid(nm * (ldb / tj), 1, S);
+  if (tj == 128)
+    hipLaunchKernelGGL(time_gemm_mfma_q_kernel, grid, dim3(QT), 0, st,
+                       (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
+  else
+    hipLaunchKernelGGL(time_gemm_mfma_q256_kernel, grid, dim3(QT), 0, st,
+                       (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
 }
 
 }  // namespace hsmd

@@ -21,6 +21,8 @@
 // exchange by permuting slot labels (configurations never move).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <cstring>
@@ -1271,6 +1273,9 @@ void mc_autotune(const Lattice &L0, int R, const double *temps, const hsmd_mc_sp
         float ms = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
         const double per = (double)ms / n_sweeps;
+        if (getenv("HSMD_TUNE_VERBOSE"))
+          fprintf(stderr, "[hsmd-tune] mc cap %d (%d colours) x %d threads: %.4f ms/sweep\n", cap,
+                  Lc.n_colours, thr, per);
         ncand++;
         if (cap == 0 && thr == kThreads) def_ms = per;
         if (per < best) { best = per; best_cap = cap; best_thr = thr; }

@@ -23,6 +23,8 @@
 // follow once at the end.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <memory>
@@ -158,7 +160,7 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
-                                int ldb, float *Y, hipStream_t st);
+                                int ldb, float *Y, hipStream_t st, int tj);
 
 struct GemmSampleHost {
   int M;
@@ -524,10 +526,10 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.f32 && e.ldt > 0 && e.gemm == 0)
+  if (e.f32 && e.ldt > 0 && e.gemm != 1)
     launch_time_gemm_mfma_wide(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw_hi.p,
                                e.tw_lo.p, e.ldt, e.chunk_base, e.chunk_fill, ldb,
-                               (float *)e.ybuf.p, 0);
+                               (float *)e.ybuf.p, 0, e.gemm == 2 ? 256 : 128);
   else if (e.f32)
     launch_time_gemm_mfma(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p,
                           (const float *)e.twid.p, ldb, e.chunk_base, e.chunk_fill,
@@ -746,18 +748,19 @@ static int current_form(const SMDEngine &e) {
 // 2 = LDS-resident, each workgroup with its own lattice's slot class (FP32
 // batches mixing two classes); 3 = global-memory form (one workgroup per
 // sample, stages in HBM / Infinity Cache).  gemm: 0 = 128 x 128 contraction
-// tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64.
+// tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64,
+// 2 = 128 x 256 with pre-split twiddles.
 // Per-site arithmetic is the same in every shape; FP64 reductions may group
 // terms differently.
 void smd_set_shape(SMDEngine &e, int form, int gemm) {
   if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
   if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
-  if (gemm < 0 || gemm > 1) throw Error(HSMD_ERR_ARG, "gemm tile must be 0 or 1");
+  if (gemm < 0 || gemm > 2) throw Error(HSMD_ERR_ARG, "gemm tile must be 0, 1 or 2");
   if ((form == 1 || form == 2) && !e.lds_ok)
     throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
   if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
-  if (gemm == 0 && !(e.f32 && e.ldt > 0))
-    throw Error(HSMD_ERR_ARG, "the 128 x 128 tile needs FP32 and 2 n_omega a multiple of 256");
+  if (gemm != 1 && !(e.f32 && e.ldt > 0))
+    throw Error(HSMD_ERR_ARG, "the 128-row MFMA tiles need FP32 and 2 n_omega a multiple of 256");
   if (form == 1 || form == 2) {
     e.use_lds = true;
     e.wide = false;
@@ -851,13 +854,14 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
       smd_set_samples(*e, spins.data());
       smd_step(*e, 1);
       const double per = timed([&] { smd_step(*e, k); }) / k;
+      if (getenv("HSMD_TUNE_VERBOSE")) fprintf(stderr, "[hsmd-tune] smd form %d: %.4f ms/step\n", f, per);
       ncand++;
       if (f == def_form) def_ms = per;
       if (per < best) { best = per; best_form = f; }
     }
     if (sp.dssf && e->f32) {
       std::vector<int> tiles = {1};
-      if (e->ldt > 0) tiles = {0, 1};
+      if (e->ldt > 0) tiles = {0, 2, 1};
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);
@@ -867,6 +871,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
         e->chunk_base = 0;
         e->chunk_fill = std::min(e->Tc, sp.n_steps);
         const double ms = timed([&] { flush_chunk(*e); });
+        if (getenv("HSMD_TUNE_VERBOSE")) fprintf(stderr, "[hsmd-tune] smd contraction tile %d: %.3f ms\n", g, ms);
         ncand++;
         if (g == def_gemm) gdef = ms;
         if (ms < gbest) { gbest = ms; best_gemm = g; }

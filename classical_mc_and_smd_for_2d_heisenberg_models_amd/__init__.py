@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(_HERE, "libhsmd.so")
 
 PLAIN_F64, BATCHED_F32, BATCHED_F64 = 0, 1, 2
 HEISENBERG, ISING = 0, 1
-N_MOMENTS = 66
+N_MOMENTS = 93
+QC_HIST = 1886
 MOMENT_NAMES = ["E", "E2", "F1", "F2", "F4", "T1", "T2", "Q1", "Q2", "Q4",
                 "S1", "S2", "S4", "SH", "SV", "SD", "X1", "X2", "X4"]
 
@@ -101,6 +102,8 @@ SIGNATURES = {
     "hsmd_lattice_bonds": (_I, [_P, _I32, _I32]),
     "hsmd_lattice_colours": (_I, [_P, _I32]),
     "hsmd_lattice_set_colour_cap": (_I, [_P, ctypes.c_int32]),
+    "hsmd_lattice_set_qc_states": (_I, [_P, _D, _D, _I, _I32]),
+    "hsmd_mc_qc_histograms": (_I, [_P, _I32, _I]),
     "hsmd_mc_set_threads": (_I, [_P, _I]),
     "hsmd_mc_autotune": (_I, [_P, _I, _D, ctypes.POINTER(MCSpec), _I, ctypes.POINTER(MCTuning)]),
     "hsmd_smd_set_shape": (_I, [_P, _I, _I]),
@@ -235,6 +238,16 @@ class Lattice:
         _check(lib().hsmd_lattice_set_impurities(self._h, _ip(imp)))
         self._refresh()
 
+    def set_qc_states(self, neel, stripes, neel_phase=False):
+        """Quasicrystal reference states (Neel n x 3, stripes 4 x n x 3) for
+        the projection order parameters; returns the eight S(q) hotspots
+        (nx, ny) in units of dq."""
+        nl = np.ascontiguousarray(neel, dtype=np.float64).reshape(self.N, 3)
+        st = np.ascontiguousarray(stripes, dtype=np.float64).reshape(4, self.N, 3)
+        pk = np.zeros((8, 2), np.int32)
+        _check(lib().hsmd_lattice_set_qc_states(self._h, _dp(nl), _dp(st), int(neel_phase), _ip(pk)))
+        return pk
+
     def set_colour_cap(self, cap):
         """Colour schedule of the GPU sweeps: 0 = default, > 0 = first-fit
         with at most cap sites per colour."""
@@ -361,6 +374,13 @@ class MonteCarlo:
         """n colour sweeps of every replica, no measurement or exchange
         (times the sweep kernel alone)."""
         _check(lib().hsmd_debug_mc_sweeps(self._h, int(n_sweeps)))
+
+    def qc_histograms(self, reset=False):
+        """Per-slot quasicrystal histograms: (neel [R, 41], stripes [R, 41, 4],
+        stripe vector [R, 41 (imag), 41 (real)])."""
+        h = np.zeros((self.R, QC_HIST), np.int32)
+        _check(lib().hsmd_mc_qc_histograms(self._h, _ip(h), int(reset)))
+        return h[:, :41], h[:, 41:205].reshape(self.R, 41, 4), h[:, 205:].reshape(self.R, 41, 41)
 
     def energies(self):
         e = np.zeros(self.R)

@@ -26,6 +26,7 @@ void mc_pt_counters(MCEngine &e, int64_t *att, int64_t *acc);
 void mc_reset_counters(MCEngine &e);
 void mc_moments(MCEngine &e, double *out, int64_t *n, int reset);
 void mc_slot_map(MCEngine &e, int32_t *m);
+void mc_qc_histograms(MCEngine &e, int32_t *out, int reset);
 void mc_set_sq(MCEngine &e, int every);
 void mc_sq_field(MCEngine &e, double *out, int64_t *count, int reset);
 void static_sq(const Lattice &L, int n_configs, const double *spins, double *out);
@@ -163,6 +164,19 @@ hsmd_status hsmd_lattice_set_colour_cap(hsmd_lattice *lat, int32_t cap) {
   });
 }
 
+hsmd_status hsmd_lattice_set_qc_states(hsmd_lattice *lat, const double *neel, const double *stripes,
+                                       int neel_phase, int32_t *peaks) {
+  return guarded([&] {
+    REQUIRE(lat && neel && stripes, "null argument");
+    lattice_set_qc(lat->lat, neel, stripes, neel_phase != 0);
+    if (peaks)
+      for (int i = 0; i < 8; i++) {
+        peaks[2 * i] = lat->lat.qc_nx[i];
+        peaks[2 * i + 1] = lat->lat.qc_ny[i];
+      }
+  });
+}
+
 hsmd_status hsmd_poisson_disk(double box, double r_min, uint64_t seed, int32_t max_points,
                               double *points, int32_t *n_points) {
   return guarded([&] {
@@ -282,6 +296,13 @@ hsmd_status hsmd_mc_autotune(const hsmd_lattice *lat, int R, const double *temps
   return guarded([&] {
     REQUIRE(lat && temps && spec && out, "null argument");
     mc_autotune(lat->lat, R, temps, *spec, n_sweeps, out);
+  });
+}
+
+hsmd_status hsmd_mc_qc_histograms(hsmd_mc *mc, int32_t *out, int reset) {
+  return guarded([&] {
+    REQUIRE(mc && out, "null argument");
+    mc_qc_histograms(*mc->e, out, reset);
   });
 }
 

@@ -27,6 +27,7 @@
 namespace hsmd {
 
 // postproc.cpp
+void write_qc_histograms(const std::string &dir, const int32_t *h, const std::string &tag);
 void write_mc_spectra(const std::string &dir, const Lattice &L, const double *sq,
                       const std::string &tagDat);
 void write_smd_spectra(const std::string &dir, const Lattice &L, const double *dssf,
@@ -40,7 +41,7 @@ namespace {
 // with_recField with_DFTcodes force_Stripes refState_Zero qct_NeelPhase
 // qct_RemovePBC rec_orderMaps
 enum { PT_ON, PT_ADAPT, PT_TRACK, ANNL_ON, FFTW_ON, RK_CHECK, PBC_OFF, ISI_MODEL, QCRYSTAL,
-       WITH_RECFIELD, WITH_DFT, FORCE_STRIPES };
+       WITH_RECFIELD, WITH_DFT, FORCE_STRIPES, REFSTATE_ZERO, QCT_NEELPHASE };
 
 constexpr int kBinSz = 1000;        // CF/subrouts.cpp:309
 constexpr double kTempMax = 2.0;    // CF/subrouts.cpp:373
@@ -420,10 +421,10 @@ std::string t_tag(double T, const std::string &label) { return "_T(" + fixed(T, 
 // Values2 (S(q) peaks and correlation lengths at the five key points, scaled
 // by 1/(measCnt2 Gsz)).  Entries with |v| <= 1e-12 (or NaN) are written as 0.
 void bin_values(const double *mom, int64_t n, int N, int Gsz, double beta, bool ising, bool c3,
-                double v1[23], double v2[40]) {
+                bool qc, double v1[23], double v2[40], double v3[45]) {
   const double fc1 = 1.0 / (double)n;
-  double a[25];
-  for (int k = 0; k < 25; k++) a[k] = fc1 * mom[k];
+  double a[HSMD_N_MOMENTS];
+  for (int k = 0; k < HSMD_N_MOMENTS; k++) a[k] = fc1 * mom[k];
   const double E1 = a[0], E2 = a[1], F1 = a[2], F2 = a[3], F4 = a[4], T1 = a[5], T2 = a[6];
   const double Q1 = a[7], Q2 = a[8], Q4 = a[9], S1 = a[10], S2 = a[11], S4 = a[12];
   const double SH = a[13], SV = a[14], SD = a[15], X1 = a[16], X2 = a[17], X4 = a[18];
@@ -442,6 +443,30 @@ void bin_values(const double *mom, int64_t n, int N, int Gsz, double beta, bool 
   }
   if (c3) XB = 1.5 * (X2 == 0 ? 0 : 1.0 - X4 / (2.0 * X2 * X2));
   else XB = 1.5 * (X2 == 0 ? 0 : 1.0 - X4 / (3.0 * X2 * X2));
+  const double fs0 = N * beta;
+  for (int k = 0; k < 45; k++) v3[k] = 0.0;
+  if (qc) {
+    // quasicrystal Binder forms (CF/mc_hm_code.cpp:5471-5495) and Values3
+    // (StPrj / SqPks / N1fpar blocks, CF/mc_hm_code.cpp:5547-5640)
+    QB = Q2 == 0 ? 0 : 1.0 - Q4 / (3.0 * Q2 * Q2);
+    SB = S2 == 0 ? 0 : 1.5 - S4 / (S2 * S2);
+    XB = 1.5 * (X2 == 0 ? 0 : 1.0 - X4 / (X2 * X2));
+    for (int k = 0; k < 4; k++) {
+      const double p1 = a[66 + k], p2 = a[70 + k], p4 = a[74 + k];
+      const double s1 = a[81 + k], s2 = a[85 + k], s4 = a[89 + k];
+      const double row1[5] = {p1, p2, p4, p2 > 0.0 ? 1.0 - p4 / (3.0 * p2 * p2) : 0.0, fs0 * (p2 - p1 * p1)};
+      const double row2[5] = {s1, s2, s4, s2 > 0.0 ? 2.0 - s4 / (s2 * s2) : 0.0, fs0 * (s2 - s1 * s1)};
+      for (int j = 0; j < 5; j++) {
+        v3[5 * k + j] = row1[j];
+        v3[20 + 5 * k + j] = row2[j];
+      }
+    }
+    const double n1 = a[78], n2 = a[79], n4 = a[80];
+    const double row3[5] = {n1, n2, n4, n2 == 0.0 ? 0.0 : 1.0 - n4 / (3.0 * n2 * n2), fs0 * (n2 - n1 * n1)};
+    for (int j = 0; j < 5; j++) v3[40 + j] = row3[j];
+    for (int k = 0; k < 45; k++)
+      if (!(fabs(v3[k]) > 1.0e-12)) v3[k] = 0.0;
+  }
   // spin stiffness, fac_RhoSt = -2 / (sqrt(3) N) (CF/mc_hm_code.cpp:3741, 5411-5415)
   const double fr = -2.0 / (sqrt(3.0) * N);
   const double RhoS1 = fr * (a[19] + a[20] + a[21]) / 3.0;
@@ -495,6 +520,26 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
   if (s.imp > 0.0) {
     imp = impurity_map(N, s.imp, seed);
     check(hsmd_lattice_set_impurities(lh.p, imp.data()));
+  }
+  const bool qc = s.code[QCRYSTAL] != 0;
+  if (qc) {
+    // reference states of the projection order parameters
+    // (CF/mc_hm_code.cpp:1747-1860; zeros with refState_Zero)
+    std::vector<double> neel((size_t)N * 3, 0.0), strp((size_t)4 * N * 3, 0.0);
+    if (!s.code[REFSTATE_ZERO]) {
+      const std::string tag = "_AppxtSz(" + std::to_string(N) + ").bin";
+      const char *names[5] = {"stateNeel0", "stateStrp1", "stateStrp2", "stateStrp3", "stateStrp4"};
+      for (int r = 0; r < 5; r++) {
+        const std::string f = s.dir + "/outBin/" + names[r] + tag;
+        long long sz = 0;
+        if (!file_size(f, sz) || sz != (long long)N * 24) fail(HSMD_ERR_IO, "Invalid data in: " + f);
+        const std::vector<double> v = read_bin<double>(f, (size_t)3 * N);
+        std::copy(v.begin(), v.end(), r == 0 ? neel.begin() : strp.begin() + (size_t)(r - 1) * N * 3);
+      }
+    } else if (s.IState == "inputf") {
+      fail(HSMD_ERR_GUARD, "Input initial qct-state is invalid, reference states must be available");
+    }
+    check(hsmd_lattice_set_qc_states(lh.p, neel.data(), strp.data(), s.code[QCT_NEELPHASE] ? 1 : 0, nullptr));
   }
   const std::vector<double> T = temperature_grid(s, npt);
   hsmd_mc_spec sp;
@@ -551,14 +596,19 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
     }
     check(hsmd_mc_moments(mc, mom.data(), &nm, 1));
     for (int k = 0; k < npt; k++) {
-      double v1[23], v2[40];
+      double v1[23], v2[40], v3[45];
       bin_values(&mom[(size_t)k * HSMD_N_MOMENTS], nm, N, Gsz, 1.0 / T[k], s.code[ISI_MODEL] != 0,
-                 c3, v1, v2);
-      const double zeros[45] = {0};
+                 c3, qc, v1, v2, v3);
       write_bin(f1[k], v1, 23, true);
       write_bin(f2[k], v2, 40, true);
-      write_bin(f3[k], zeros, 45, true);
+      write_bin(f3[k], v3, 45, true);
     }
+  }
+  if (qc) {
+    std::vector<int32_t> hist((size_t)npt * HSMD_QC_HIST);
+    check(hsmd_mc_qc_histograms(mc, hist.data(), 0));
+    for (int k = 0; k < npt; k++)
+      write_qc_histograms(dir, &hist[(size_t)k * HSMD_QC_HIST], t_tag(T[k], s.label));
   }
   if (dft) {
     std::vector<double> sqf((size_t)npt * Nq * 6);

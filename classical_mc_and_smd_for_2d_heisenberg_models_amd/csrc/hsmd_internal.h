@@ -56,6 +56,12 @@ struct Lattice {
   // colour schedule: 0 = default (first-fit, rebalanced when cheap);
   // > 0 = first-fit with at most col_cap sites per colour (autotuner)
   int col_cap = 0;
+  // quasicrystal reference states (qc.cpp): 5 x N x 3 in original site
+  // order (Neel, stripes 1-4) and the eight S(q) hotspots (nx, ny) in units
+  // of sq_dq(); empty when not set
+  std::vector<double> qc_ref;
+  int qc_nx[8] = {0, 0, 0, 0, 0, 0, 0, 0}, qc_ny[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool qc_on() const { return !qc_ref.empty(); }
 
   // --- derived (rebuild())
   std::vector<int32_t> bi, bj, bshell;  // reference bond list (make_bondList order)
@@ -99,6 +105,9 @@ Lattice *lattice_generate(const std::string &geom, int L, bool pbc, bool strict,
 Lattice *lattice_from_tables(int n, int zn1, int zn2, int znx, const int32_t *nb,
                              const double *rvec, const hsmd_couplings &c);
 void derive_flags(Lattice &lat);
+
+// qc.cpp: quasicrystal reference states and hotspots
+void lattice_set_qc(Lattice &L, const double *neel, const double *strp, bool neel_phase);
 
 // tune.cpp: signature hashing and the optional on-disk tuning cache
 // (path in the environment variable HSMD_TUNE_CACHE; no caching when unset)

@@ -39,6 +39,11 @@ constexpr int NOBSX = 40;  // per-configuration sums of obs_kernel: order
                            // parameters [0..31], stiffness C-sums [32..37]
 constexpr int NMOM = HSMD_N_MOMENTS;
 constexpr int kThreads = 1024;
+// quasicrystal measurement: raw sums per configuration (qc_obs_kernel) and
+// per-slot histograms (Neel 41, stripes 41 x 4, stripe vector 41 x 41)
+constexpr int NQC = 56;
+constexpr int kHst = 41;
+constexpr int kQcHist = HSMD_QC_HIST;
 
 struct SweepArgs {
   Coup cp;
@@ -68,6 +73,12 @@ struct SweepArgs {
   double *obs;             // per configuration, NOBSX
   const double *stiff_w;   // 6 x Nb: J C0^2, J C1^2, J C2^2, J C0, J C1, J C2
   const uint32_t *sweep_dev;   // graph replays: sweep index read here instead of `sweep`
+  // quasicrystal measurement (qc != 0): reference states [5][3][N], hotspot
+  // phases e^{-i q_i.r} [8][N], shell-1 slot count, sums out [NQC] per config
+  int qc, qc_zn1;
+  const double *qc_ref;
+  const double2 *qc_phase;
+  double *qc_obs;
 };
 
 __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
@@ -283,6 +294,83 @@ __global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NOBSX; k++) a.obs[(size_t)c * NOBSX + k] = v[k];
+  }
+}
+
+// Quasicrystal order parameters of one configuration (one workgroup), as
+// raw site sums: projections on the Neel and four stripe reference states
+// (get_qctNeelPrjt / get_qctStrpPrjt, CF/subrouts.cpp:6053-6180), the
+// clock-ordered stripe vector (get_qctStripeMag, :6183-6262), the
+// nearest-neighbour frustration parameter (get_qctNeel_N1fpar, :6011-6050)
+// and the S(q) sums at the eight hotspots (get_qct_SqPks4, :8637-8700):
+//   [0] sum S.neel, [1..4] sum S.stripe_i, [5, 6] sum rfac (rx, ry),
+//   [7] sum (1 - S.<S_nb>) / 2, [8 + 6 i + 2 d] / [.. + 1] Re / Im of
+//   sum S_d e^{-i q_i.r}.
+// rx, ry: the site's spin against the differences of the clock-ordered
+// shell-1 neighbours (slots 0..7, missing = 0) over their count (0 when the
+// site has none; the reference divides by zero there), clipped to length 1.
+template <typename T>
+__global__ __launch_bounds__(256) void qc_obs_kernel(const SweepArgs a) {
+  __shared__ double scratch[NQC * 16];
+  const int c = blockIdx.x;
+  const int N = a.cp.N, Npad = a.cp.Npad;
+  const int comps = sizeof(T) == 1 ? 1 : 3;
+  const T *g = (const T *)a.spins + (size_t)c * comps * Npad;
+  const double *rf = a.qc_ref;
+  double v[NQC];
+#pragma unroll
+  for (int k = 0; k < NQC; k++) v[k] = 0.0;
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
+    double x, y, z;
+    load_spin<T>(g, Npad, p, x, y, z);
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+      v[r] += x * rf[(size_t)(3 * r) * N + p] + y * rf[(size_t)(3 * r + 1) * N + p] +
+              z * rf[(size_t)(3 * r + 2) * N + p];
+    double nx[8], ny[8], nz[8];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int zc = 0;
+#pragma unroll
+    for (int s = 0; s < 8; s++) { nx[s] = 0.0; ny[s] = 0.0; nz[s] = 0.0; }
+    for (int s = 0; s < a.qc_zn1; s++) {
+      const int q = a.nb[(size_t)s * N + p];
+      double bx = 0.0, by = 0.0, bz = 0.0;
+      if (q < N) {
+        load_spin<T>(g, Npad, q, bx, by, bz);
+        zc++;
+      }
+      ax += bx; ay += by; az += bz;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k == s) { nx[k] = bx; ny[k] = by; nz[k] = bz; }
+    }
+    const double fcz = zc > 0 ? 1.0 / zc : 0.0;
+    const double v1x = (nx[0] + nx[1] + nx[4] + nx[5]) - (nx[2] + nx[3] + nx[6] + nx[7]);
+    const double v1y = (ny[0] + ny[1] + ny[4] + ny[5]) - (ny[2] + ny[3] + ny[6] + ny[7]);
+    const double v1z = (nz[0] + nz[1] + nz[4] + nz[5]) - (nz[2] + nz[3] + nz[6] + nz[7]);
+    const double v2x = (nx[1] + nx[2] + nx[5] + nx[6]) - (nx[3] + nx[4] + nx[7] + nx[0]);
+    const double v2y = (ny[1] + ny[2] + ny[5] + ny[6]) - (ny[3] + ny[4] + ny[7] + ny[0]);
+    const double v2z = (nz[1] + nz[2] + nz[5] + nz[6]) - (nz[3] + nz[4] + nz[7] + nz[0]);
+    const double rx = fcz * (x * v1x + y * v1y + z * v1z);
+    const double ry = fcz * (x * v2x + y * v2y + z * v2z);
+    const double rn = sqrt(rx * rx + ry * ry);
+    const double rfac = rn > 1.0 ? 1.0 / rn : 1.0;
+    v[5] += rfac * rx;
+    v[6] += rfac * ry;
+    const double z1 = 1.0 / (double)max(1, zc);
+    v[7] += 0.5 * (1.0 - (x * z1 * ax + y * z1 * ay + z * z1 * az));
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const double2 e = a.qc_phase[(size_t)i * N + p];
+      v[8 + 6 * i + 0] += x * e.x; v[8 + 6 * i + 1] += x * e.y;
+      v[8 + 6 * i + 2] += y * e.x; v[8 + 6 * i + 3] += y * e.y;
+      v[8 + 6 * i + 4] += z * e.x; v[8 + 6 * i + 5] += z * e.y;
+    }
+  }
+  block_sum<NQC>(v, scratch);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NQC; k++) a.qc_obs[(size_t)c * NQC + k] = v[k];
   }
 }
 
@@ -521,7 +609,19 @@ struct PTArgs {
   double *etrace;          // per slot energies of this sweep (nullable)
   uint32_t *sweep_dev;     // graph replays: sweep index (advanced by this kernel)
   int parity_alt;          // with sweep_dev: pair parity = sweep % 2
+  // quasicrystal measurement: sums from qc_obs_kernel, 0.5 / Nq of
+  // get_qct_SqPks4, per-slot histograms
+  int qc;
+  double qc_fc;
+  const double *qc_obs;
+  int32_t *qc_hist;
 };
+
+// get_HistIndex (CF/subrouts.cpp:795-812) with r0 = 1 and fc = (kHst - 1) / 2
+__device__ __forceinline__ int hist_index(double r) {
+  const int i = (int)round((r + 1.0) * (0.5 * (kHst - 1)));
+  return i < 0 ? 0 : (i >= kHst ? kHst - 1 : i);
+}
 
 __device__ double stag_mag(const double *sub, int n0, double iN) {
   // absStaggMag (CF/subrouts.cpp:5382-5462)
@@ -554,7 +654,51 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
       double T1 = (a.ising ? 0.0 : o[2]) * iN;
       if (!a.h_on) T1 = fabs(T1);
       double Q1 = 0, SH = 0, SV = 0, SD = 0, X1 = 0;
-      if (a.has_labels) {
+      double S1q = -1.0;
+      if (a.qc) {
+        // quasicrystal forms (CF/mc_hm_code.cpp:4973-5056): Q = |Neel
+        // projection|, S = |stripe projections|, X = |stripe vector|
+        const double *q = a.qc_obs + (size_t)c * NQC;
+        const double P0 = iN * q[0];
+        double P[4];
+        bool cut = false;
+        int kmax = 0;
+        for (int i = 0; i < 4; i++) {
+          P[i] = iN * q[1 + i];
+          if (fabs(fabs(P[i]) - 1.0) < 1.0e-12) { kmax = i; cut = true; }
+        }
+        if (cut)
+          for (int i = 0; i < 4; i++)
+            if (i != kmax) P[i] = 0.0;
+        Q1 = fabs(P0);
+        S1q = sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2] + P[3] * P[3]);
+        const double qx = iN * q[5], qy = iN * q[6];
+        X1 = sqrt(qx * qx + qy * qy);
+        const double N1 = fabs(iN * q[7]);
+        double pk[4];
+        for (int j = 0; j < 4; j++) {
+          double sv = 0.0;
+          for (int h = 0; h < 2; h++) {
+            const double *z = q + 8 + 6 * (2 * j + h);
+            sv += (1.0 / 3.0) * (z[0] * z[0] + z[1] * z[1] + z[2] * z[2] + z[3] * z[3] +
+                                 z[4] * z[4] + z[5] * z[5]);
+          }
+          pk[j] = sqrt(a.qc_fc * sv);
+        }
+        for (int i = 0; i < 4; i++) {
+          const double v1 = fabs(P[i]), v2 = v1 * v1;
+          m[66 + i] += v1; m[70 + i] += v2; m[74 + i] += v2 * v2;
+          const double w2 = pk[i] * pk[i];
+          m[81 + i] += pk[i]; m[85 + i] += w2; m[89 + i] += w2 * w2;
+        }
+        m[78] += N1; m[79] += N1 * N1; m[80] += N1 * N1 * N1 * N1;
+        // histograms (CF/mc_hm_code.cpp:5004-5028)
+        int32_t *hg = a.qc_hist + (size_t)k * kQcHist;
+        hg[hist_index(P0)] += 1;
+        for (int i = 0; i < 4; i++) hg[kHst + 4 * hist_index(P[i]) + i] += 1;
+        const double pre = fabs(P[0]) - fabs(P[2]), pim = fabs(P[1]) - fabs(P[3]);
+        hg[5 * kHst + kHst * hist_index(pim) + hist_index(pre)] += 1;
+      } else if (a.has_labels) {
         Q1 = stag_mag(o + 3, a.n_stag, iN);
         SH = stag_mag(o + 12, 2, iN);
         SV = stag_mag(o + 18, 2, iN);
@@ -569,7 +713,7 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
           X1 = iN * o[30];
         }
       }
-      const double S1 = sqrt(SH * SH + SV * SV + (a.c3 ? SD * SD : 0.0));
+      const double S1 = S1q >= 0.0 ? S1q : sqrt(SH * SH + SV * SV + (a.c3 ? SD * SD : 0.0));
       const double Q2 = Q1 * Q1, S2 = S1 * S1, X2 = X1 * X1;
       m[0] += E; m[1] += E * E;
       m[2] += F1; m[3] += F2; m[4] += F2 * F2;
@@ -659,6 +803,10 @@ struct MCEngine {
   DevBuf<int32_t> cfg_of_slot, slot_of_cfg, boost;
   DevBuf<long long> pt_cnt, pt_exc, n_meas;
   DevBuf<double> stiff_w;
+  // quasicrystal measurement
+  DevBuf<double> qc_ref, qc_obs;
+  DevBuf<double2> qc_phase;
+  DevBuf<int32_t> qc_hist;
   // static S(q) measurement
   int sq_every = 0;
   long long meas_count = 0;   // measured sweeps since the last moments reset
@@ -804,6 +952,27 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
     e->stiff_w.upload(w.data(), w.size());
   }
   e->moments.alloc((size_t)R * NMOM); e->moments.zero();
+  if (lat.qc_on()) {
+    // reference states and hotspot phases in the engine's labels
+    std::vector<double> ref((size_t)15 * N);
+    std::vector<double2> ph((size_t)8 * N);
+    const double dq = lat.sq_dq();
+    for (int p = 0; p < N; p++) {
+      const int i = lat.perm[p];
+      for (int r = 0; r < 5; r++)
+        for (int d = 0; d < 3; d++) ref[(size_t)(3 * r + d) * N + p] = lat.qc_ref[((size_t)r * N + i) * 3 + d];
+      for (int q = 0; q < 8; q++) {
+        const double th = -dq * (lat.qc_nx[q] * lat.rvec[2 * (size_t)i] + lat.qc_ny[q] * lat.rvec[2 * (size_t)i + 1]);
+        ph[(size_t)q * N + p] = make_double2(cos(th), sin(th));
+      }
+    }
+    e->qc_ref.upload(ref.data(), ref.size());
+    e->qc_phase.upload(ph.data(), ph.size());
+    e->qc_obs.alloc((size_t)R * NQC);
+    e->qc_obs.zero();
+    e->qc_hist.alloc((size_t)R * kQcHist);
+    e->qc_hist.zero();
+  }
   std::vector<int32_t> id(R);
   for (int k = 0; k < R; k++) id[k] = k;
   e->cfg_of_slot.upload(id.data(), R);
@@ -838,6 +1007,11 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
   a.energy = e->energy.p;
   a.obs = e->obs.p;
   a.stiff_w = e->stiff_w.p;
+  a.qc = lat.qc_on() ? 1 : 0;
+  a.qc_zn1 = lat.zn1;
+  a.qc_ref = e->qc_ref.p;
+  a.qc_phase = e->qc_phase.p;
+  a.qc_obs = e->qc_obs.p;
   {
     int maxcol = 0;
     for (int k = 0; k < lat.n_colours; k++) maxcol = std::max(maxcol, lat.col_off[k + 1] - lat.col_off[k]);
@@ -981,6 +1155,12 @@ static void launch_obs(MCEngine &e, hipStream_t st) {
   else if (e.prec == HSMD_BATCHED_F32)
     hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, st, e.args);
   else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, st, e.args);
+  if (e.args.qc) {
+    if (e.ising) hipLaunchKernelGGL(qc_obs_kernel<int8_t>, dim3(e.R), dim3(256), 0, st, e.args);
+    else if (e.prec == HSMD_BATCHED_F32)
+      hipLaunchKernelGGL(qc_obs_kernel<float>, dim3(e.R), dim3(256), 0, st, e.args);
+    else hipLaunchKernelGGL(qc_obs_kernel<double>, dim3(e.R), dim3(256), 0, st, e.args);
+  }
 }
 
 // Sweep loop as a replayed hipGraph: the graph of one sweep (sweep kernel,
@@ -1047,6 +1227,10 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
   p.config_of_slot = e.cfg_of_slot.p; p.slot_of_config = e.slot_of_cfg.p;
   p.boost_slot = e.boost.p; p.pt_cnt = e.pt_cnt.p; p.pt_exc = e.pt_exc.p;
   p.moments = e.moments.p; p.n_meas = e.n_meas.p;
+  p.qc = e.args.qc;
+  p.qc_fc = 0.5 / (double)e.lat.Nq;
+  p.qc_obs = e.qc_obs.p;
+  p.qc_hist = e.qc_hist.p;
   e.args.measure = measure;
   const int pthreads = ((e.R + 63) / 64) * 64;
   // the measurement / exchange kernel is skipped when it has nothing to do
@@ -1152,6 +1336,12 @@ void mc_sq_field(MCEngine &e, double *out, int64_t *count, int reset) {
                         hipMemcpyDeviceToHost));
   if (count) *count = e.sq_count;
   if (reset) { e.sqfield.zero(); e.sq_count = 0; }
+}
+
+void mc_qc_histograms(MCEngine &e, int32_t *out, int reset) {
+  if (!e.args.qc) throw Error(HSMD_ERR_STATE, "the lattice has no quasicrystal reference states");
+  HIP_CHECK(hipMemcpy(out, e.qc_hist.p, (size_t)e.R * kQcHist * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (reset) e.qc_hist.zero();
 }
 
 void mc_slot_map(MCEngine &e, int32_t *m) {

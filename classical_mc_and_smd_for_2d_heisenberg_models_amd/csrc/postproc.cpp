@@ -8,6 +8,7 @@
 //   SMD files                 CF/smd_code.cpp:3628-4130
 // Host-only I/O; the spectra come from the GPU engines.
 #include <math.h>
+#include <stdio.h>
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -284,6 +285,66 @@ void write_smd_spectra(const std::string &dir, const Lattice &L, const double *d
     }
     for (int f = 0; f < 4; f++) out[f] << "\n";
     cut << "\n";
+  }
+}
+
+// Quasicrystal histograms of one temperature (CF/mc_hm_code.cpp:5961-6130):
+// counts h[HSMD_QC_HIST] (Neel [41], stripes [41][4], stripe vector
+// [41 imag][41 real]) added to the counts already in the binary files
+// MC_Bin_NeelHist / StrpHist / SMagVecHist<tag>.bin (a file of another size
+// is an error, as in the reference), written back, and the normalised
+// histograms as outData/NeelHist / StrpHist / SMagVecHist<tag>.dat.
+void write_qc_histograms(const std::string &dir, const int32_t *h, const std::string &tag) {
+  constexpr int H1 = 41;
+  const double box = 2.0 / (H1 - 1);
+  std::vector<int32_t> neel(h, h + H1), strp(h + H1, h + 5 * H1), vec(h + 5 * H1, h + 5 * H1 + H1 * H1);
+  auto merge = [&](const std::string &name, std::vector<int32_t> &v) {
+    const std::string path = dir + "/outBin/" + name + tag + ".bin";
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
+    if (in) {
+      const long long sz = (long long)in.tellg();
+      if (sz != (long long)(v.size() * sizeof(int32_t)))
+        throw Error(HSMD_ERR_IO, "histogram file of the wrong size: " + path);
+      in.seekg(0);
+      std::vector<int32_t> old(v.size());
+      in.read(reinterpret_cast<char *>(old.data()), sz);
+      for (size_t i = 0; i < v.size(); i++) v[i] += old[i];
+    }
+    std::ofstream out(path, std::ios::binary | std::ios::trunc);
+    if (!out) throw Error(HSMD_ERR_IO, "cannot write " + path);
+    out.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(int32_t)));
+  };
+  merge("MC_Bin_NeelHist", neel);
+  merge("MC_Bin_StrpHist", strp);
+  merge("MC_Bin_SMagVecHist", vec);
+  auto fix = [](double x) {
+    char b[64];
+    snprintf(b, sizeof b, "%+8.4f", x);
+    return std::string(b);
+  };
+  long long n0 = 0, n4[4] = {0, 0, 0, 0}, nv = 0;
+  for (int i = 0; i < H1; i++) {
+    n0 += neel[i];
+    for (int k = 0; k < 4; k++) n4[k] += strp[4 * i + k];
+  }
+  for (int v : vec) nv += v;
+  const std::string base = dir + "/outData/";
+  std::ofstream o1(base + "NeelHist" + tag + ".dat"), o2(base + "StrpHist" + tag + ".dat");
+  if (!o1 || !o2) throw Error(HSMD_ERR_IO, "cannot write histogram files in " + base);
+  for (int i = 0; i < H1; i++) {
+    const double x = i * box - 1.0;
+    o1 << fix(x) << "  " << fmt_sci(neel[i] / (double)n0, 6, 14) << "  \n";
+    o2 << fix(x) << "  ";
+    for (int k = 0; k < 4; k++) o2 << fmt_sci(strp[4 * i + k] / (double)n4[k], 6, 14) << "  ";
+    o2 << "\n";
+  }
+  std::ofstream o3(base + "SMagVecHist" + tag + ".dat");
+  if (!o3) throw Error(HSMD_ERR_IO, "cannot write histogram files in " + base);
+  for (int i = 0; i < H1; i++) {
+    for (int j = 0; j < H1; j++)
+      o3 << fix(i * box - 1.0) << "  " << fix(j * box - 1.0) << "  "
+         << fmt_sci(vec[(size_t)j * H1 + i] / (double)nv, 6, 14) << "\n";
+    o3 << "\n";
   }
 }
 

@@ -131,6 +131,16 @@ hsmd_status hsmd_lattice_colours(const hsmd_lattice *lat, int32_t *colour);
  * colours); cap > 0 is first-fit with at most cap sites per colour.  Engines
  * created afterwards use it; hsmd_mc_autotune picks the cap. */
 hsmd_status hsmd_lattice_set_colour_cap(hsmd_lattice *lat, int32_t cap);
+/* Quasicrystal reference states for the projection order parameters:
+ * neel[n x 3], stripes[4 x n x 3] (stateNeel0 / stateStrp1..4_AppxtSz(n).bin,
+ * read at CF/mc_hm_code.cpp:1747-1860); the S(q) hotspots are located from
+ * the Neel state (neel_phase = 1) or the first stripe state (qct_NeelPhase,
+ * get_qct_SqPkLoc CF/subrouts.cpp:8531).  MC engines created afterwards
+ * measure the quasicrystal observables.  peaks (nullable): 8 x 2 int32
+ * (nx, ny) in units of the lattice's dq. */
+hsmd_status hsmd_lattice_set_qc_states(hsmd_lattice *lat, const double *neel,
+                                       const double *stripes, int neel_phase,
+                                       int32_t *peaks);
 void hsmd_lattice_destroy(hsmd_lattice *lat);
 
 /* --------------------------------------------------- fields and energies */
@@ -217,10 +227,23 @@ hsmd_status hsmd_mc_reset_counters(hsmd_mc *mc);
  *            xyCLen..ttCLen, Values2[0..19]);
  *   [45..64] correlation lengths (get_corrLen's second output, xySVal..ttSVal,
  *            Values2[20..39]);
- *   [65]     number of S(q) measurements (measCnt2).
+ *   [65]     number of S(q) measurements (measCnt2);
+ *   [66..92] quasicrystal lattices with reference states
+ *            (hsmd_lattice_set_qc_states; CF/mc_hm_code.cpp:4973-5160):
+ *            StPrjVc1[4], StPrjVc2[4], StPrjVc4[4], N1fpar 1/2/4,
+ *            SqPks4P1[4], SqPks4P2[4], SqPks4P4[4].  On such lattices
+ *            Q, S, X of [7..18] are the projection forms (|Neel projection|,
+ *            |stripe projections|, |stripe vector| of get_qctStripeMag).
  * [25..65] accumulate only when S(q) measurement is on (hsmd_mc_set_sq). */
-#define HSMD_N_MOMENTS 66
+#define HSMD_N_MOMENTS 93
+/* Per-slot quasicrystal histograms (int32): Neel projection [41], stripe
+ * projections [41][4], stripe vector [41 (imag)][41 (real)]
+ * (NeelHist / StrpHist / SMagVecHist, CF/mc_hm_code.cpp:4496-4745). */
+#define HSMD_QC_HIST 1886
 hsmd_status hsmd_mc_moments(hsmd_mc *mc, double *out, int64_t *n_meas, int reset);
+/* The quasicrystal histograms per slot, out[R x HSMD_QC_HIST] (counts since
+ * the last reset; they are not cleared by hsmd_mc_moments). */
+hsmd_status hsmd_mc_qc_histograms(hsmd_mc *mc, int32_t *out, int reset);
 /* Static S(q) measurement during measured sweeps: at every `every`-th
  * measured sweep counted from the last hsmd_mc_moments reset, starting with
  * the first (the reference's nbin % dftDelay1 == 0 within a bin,

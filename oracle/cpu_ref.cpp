@@ -1083,4 +1083,125 @@ void orc_corr_len(int G, int c3, double dq, const double *sq, double *peaks, dou
     }
 }
 
+// Quasicrystal order parameters of R configurations (per configuration:
+// [0] NeelPrjt, [1..4] StrpPrjt after the cut to the saturated state,
+// [5] QctMag, [6] the signed Neel_N1fpar w0, [7..10] SqPeaks).
+//  get_qctNeelPrjt    CF/subrouts.cpp:6053-6084
+//  get_qctStrpPrjt    CF/subrouts.cpp:6086-6180 (cut: |P_k| within 1e-12 of 1)
+//  get_qctStripeMag   CF/subrouts.cpp:6183-6262 (fcz = 1/znum1; 0 here when a
+//                     site has no shell-1 neighbour, where the reference divides by 0)
+//  get_qctNeel_N1fpar CF/subrouts.cpp:6011-6050
+//  get_qct_SqPks4     CF/subrouts.cpp:8637-8700
+// ref: 5 x N x 3 (Neel, stripes 1-4); pk: 8 x (nx, ny); nbors: N x Zt with
+// shell 1 in columns 0..zn1-1 (-1 = missing).
+void orc_qc_values(int N, const int32_t *nbors, int Zt, int zn1, int R, const double *spins,
+                   const double *ref, const double *rvec, int Nq, double dq, const int32_t *pk,
+                   double *out) {
+  const double iNs = 1.0 / N;
+  for (int c = 0; c < R; c++) {
+    const double *S = spins + (size_t)c * N * 3;
+    double *o = out + (size_t)c * 11;
+    double pr[5] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < 5; r++) {
+      double acc = 0.0;
+      for (int k = 0; k < N; k++) {
+        const double *rs = ref + ((size_t)r * N + k) * 3;
+        acc += S[3 * k] * rs[0] + S[3 * k + 1] * rs[1] + S[3 * k + 2] * rs[2];
+      }
+      pr[r] = iNs * acc;
+    }
+    o[0] = pr[0];
+    bool cut = false;
+    int kmax = 0;
+    for (int k = 0; k < 4; k++)
+      if (std::abs(std::abs(pr[1 + k]) - 1.0) < 1.0e-12) { kmax = k; cut = true; }
+    for (int k = 0; k < 4; k++) o[1 + k] = (cut && k != kmax) ? 0.0 : pr[1 + k];
+    // stripe vector
+    double m1 = 0.0, m2 = 0.0, fpar = 0.0;
+    for (int i = 0; i < N; i++) {
+      double set[8][3];
+      int zc = 0;
+      double sum[3] = {0, 0, 0};
+      for (int j = 0; j < zn1; j++) {
+        const int nb = nbors[(size_t)i * Zt + j];
+        double v[3] = {0, 0, 0};
+        if (nb >= 0) {
+          zc++;
+          for (int d = 0; d < 3; d++) v[d] = S[3 * nb + d];
+        }
+        for (int d = 0; d < 3; d++) sum[d] += v[d];
+        if (j < 8)
+          for (int d = 0; d < 3; d++) set[j][d] = v[d];
+      }
+      for (int j = zn1; j < 8; j++)
+        for (int d = 0; d < 3; d++) set[j][d] = 0.0;
+      double rx = 0.0, ry = 0.0;
+      const double fcz = zc > 0 ? 1.0 / zc : 0.0;
+      for (int d = 0; d < 3; d++) {
+        const double v1 = (set[0][d] + set[1][d] + set[4][d] + set[5][d]) -
+                          (set[2][d] + set[3][d] + set[6][d] + set[7][d]);
+        const double v2 = (set[1][d] + set[2][d] + set[5][d] + set[6][d]) -
+                          (set[3][d] + set[4][d] + set[7][d] + set[0][d]);
+        rx += S[3 * i + d] * v1;
+        ry += S[3 * i + d] * v2;
+      }
+      rx *= fcz;
+      ry *= fcz;
+      const double rn = sqrt(rx * rx + ry * ry);
+      const double rf = rn > 1.0 ? 1.0 / rn : 1.0;
+      m1 += rf * rx;
+      m2 += rf * ry;
+      const double z1 = 1.0 / std::max(1, zc);
+      double dot = 0.0;
+      for (int d = 0; d < 3; d++) dot += S[3 * i + d] * (z1 * sum[d]);
+      fpar += 0.5 * (1.0 - dot);
+    }
+    o[5] = sqrt((iNs * m1) * (iNs * m1) + (iNs * m2) * (iNs * m2));
+    o[6] = iNs * fpar;
+    // S(q) at the eight hotspots
+    double spec[8];
+    for (int h = 0; h < 8; h++) {
+      std::complex<double> sq[3];
+      for (int k = 0; k < N; k++) {
+        const double ph = -dq * (pk[2 * h] * rvec[2 * k] + pk[2 * h + 1] * rvec[2 * k + 1]);
+        const std::complex<double> e(cos(ph), sin(ph));
+        for (int d = 0; d < 3; d++) sq[d] += S[3 * k + d] * e;
+      }
+      spec[h] = (1.0 / 3.0) * (std::norm(sq[0]) + std::norm(sq[1]) + std::norm(sq[2]));
+    }
+    const double fc = 0.5 / Nq;
+    for (int j = 0; j < 4; j++) o[7 + j] = sqrt(fc * (spec[2 * j] + spec[2 * j + 1]));
+  }
+}
+
+// get_qct_SqPkLoc (CF/subrouts.cpp:8531-8632): the hotspots of a reference
+// state on the Lsz x Lsz direct-sum grid; pk = 8 x (nx, ny).
+void orc_qc_peaks(int N, const double *state, const double *rvec, int Lsz, double dq, int32_t *pk) {
+  const int l2 = Lsz / 2;
+  double best = 0.0, qxm = 0.0, qym = 0.0;
+  for (int j = 0; j < Lsz; j++) {
+    const int ny = j < l2 ? j : j - Lsz;
+    for (int i = 0; i < Lsz; i++) {
+      const int nx = i < l2 ? i : i - Lsz;
+      const double qx = nx * dq, qy = ny * dq;
+      const double th = atan2(qy, qx), qr = sqrt(qx * qx + qy * qy);
+      if (!(qr > 2.0 && qr < 5.0 && th > 0.0 && th < PI / 4.0)) continue;
+      std::complex<double> sq[3];
+      for (int k = 0; k < N; k++) {
+        const std::complex<double> e = std::exp(std::complex<double>(0.0, -dq) *
+                                                (double)(nx * rvec[2 * k] + ny * rvec[2 * k + 1]));
+        for (int d = 0; d < 3; d++) sq[d] += state[3 * k + d] * e;
+      }
+      const double v = std::norm(sq[0]) + std::norm(sq[1]) + std::norm(sq[2]);
+      if (v > best) { best = v; qxm = qx; qym = qy; }
+    }
+  }
+  const double qx[8] = {qxm, -qxm, qym, -qym, -qxm, qym, qxm, -qym};
+  const double qy[8] = {qym, -qym, qxm, -qxm, qym, -qxm, -qym, qxm};
+  for (int i = 0; i < 8; i++) {
+    pk[2 * i] = (int)std::round(qx[i] / dq);
+    pk[2 * i + 1] = (int)std::round(qy[i] / dq);
+  }
+}
+
 }  // extern "C"

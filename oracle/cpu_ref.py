@@ -47,6 +47,8 @@ _SIG = {
     "orc_corr_len": (None, [_I, _I, ctypes.c_double, _D, _D, _D]),
     "orc_dssf_direct": (None, [_I, _D, _I32, _I, _D, _I, _D, _D, _I, _D]),
     "orc_observables": (None, [ctypes.c_char_p, _I, _I, _I32, _I, _I, _I, _I, _D, _D]),
+    "orc_qc_values": (None, [_I, _I32, _I, _I, _I, _D, _D, _D, _I, ctypes.c_double, _I32, _D]),
+    "orc_qc_peaks": (None, [_I, _D, _D, _I, ctypes.c_double, _I32]),
 }
 
 _lib = None
@@ -229,6 +231,29 @@ def corr_len(sq, G, c3, dq):
     cl = np.zeros((4, 5))
     lib().orc_corr_len(G, int(c3), dq, _dp(z), _dp(pk), _dp(cl))
     return pk, cl
+
+
+def qc_values(nbors, zn1, spins, ref, rvec, Nq, dq, peaks):
+    """Quasicrystal order parameters per configuration (R, 11): NeelPrjt,
+    StrpPrjt[4] (after the cut), QctMag, signed N1fpar, SqPeaks[4]."""
+    nb = np.ascontiguousarray(nbors, dtype=np.int32)
+    s = np.ascontiguousarray(spins, dtype=np.float64).reshape(-1, nb.shape[0], 3)
+    rf = np.ascontiguousarray(ref, dtype=np.float64).reshape(5, nb.shape[0], 3)
+    rv = np.ascontiguousarray(rvec, dtype=np.float64)
+    pk = np.ascontiguousarray(peaks, dtype=np.int32).reshape(8, 2)
+    out = np.zeros((s.shape[0], 11))
+    lib().orc_qc_values(nb.shape[0], nb.ctypes.data_as(_I32), nb.shape[1], zn1, s.shape[0], _dp(s),
+                        _dp(rf), _dp(rv), Nq, dq, pk.ctypes.data_as(_I32), _dp(out))
+    return out
+
+
+def qc_peaks(state, rvec, Lsz, dq):
+    """get_qct_SqPkLoc: the eight hotspots (8, 2) of a reference state."""
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    rv = np.ascontiguousarray(rvec, dtype=np.float64)
+    pk = np.zeros((8, 2), np.int32)
+    lib().orc_qc_peaks(st.shape[0], _dp(st), _dp(rv), Lsz, dq, pk.ctypes.data_as(_I32))
+    return pk
 
 
 def dssf_direct(traj, rvec, q, window, n_omega, mask=None):

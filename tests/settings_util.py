@@ -33,6 +33,6 @@ def default(d, **kw):
     smd = [kw.get("ntm", 1280), kw.get("tpLstSz", 4), kw.get("tpIndex", 1), kw.get("dtm", 0.05),
            kw.get("win", "nowin"), 0, 0]
     dis = [kw.get("imp", 0.0), 1.0, 1.0, 0.0, 1.0, 0.0, 0.0]
-    code = [kw.get("ptON", 1), 0, 0, 0, 0, 0, 0, kw.get("ising", 0), 0, kw.get("rec", 1), 1,
-            0, 0, 0, 0, 0]
+    code = [kw.get("ptON", 1), 0, 0, 0, 0, 0, 0, kw.get("ising", 0), kw.get("qc", 0), kw.get("rec", 1),
+            kw.get("dft", 1), 0, kw.get("refzero", 0), kw.get("neelphase", 0), 0, 0]
     write_settings(d, model, mc, smd, dis, code)

@@ -169,3 +169,48 @@ def test_inputf_ferromagnet_stays_ordered_at_low_temperature(tmp_path):
         v1 = v1.reshape(-1, 23)
         assert np.all(v1[:, 15] > 0.98)      # |M|/N
         assert np.all(v1[:, 0] < -1.95)      # E/N close to -2 J1 ... J1 = -1
+
+
+@pytest.mark.gpu
+def test_quasicrystal_run_writes_values3_and_histograms(tmp_path):
+    # a tabulated quasicrystal run (qcrystal = 1, the 239-site fixture with its
+    # reference states): Values3 holds the StPrj / SqPks / N1fpar blocks with
+    # Binder cumulants consistent with their moments, and the three
+    # histograms count every measured sweep, accumulating over runs
+    import shutil
+    d = str(tmp_path)
+    default(d, L=239, geom="Aamman", qc=1, J1=1.0, J2=0.4, JX=0.1, T1=0.5, T2=2.0, NTerm=100,
+            NMeas=2000, crSkip=1000, rec=0, dft=0, ptON=1)
+    src = os.path.join(os.path.dirname(__file__), "golden", "qc_no_PBC_239")
+    for f in os.listdir(src):
+        if f.endswith(".bin"):
+            shutil.copy(os.path.join(src, f), os.path.join(d, "outBin", f))
+    os.makedirs(os.path.join(d, "outData"), exist_ok=True)
+    H.run_mc(d, 2, precision=H.BATCHED_F64, seed=5)
+    ob = os.path.join(d, "outBin")
+    v3f = sorted(f for f in os.listdir(ob) if f.startswith("MC_Bin_AvgValue3"))
+    assert len(v3f) == 2
+    for f in v3f:
+        v3 = np.fromfile(os.path.join(ob, f)).reshape(-1, 45)
+        assert v3.shape[0] == 2
+        for b in range(2):
+            for k in range(4):
+                p1, p2, p4, bc = v3[b, 5 * k:5 * k + 4]
+                assert 0.0 <= p1 <= 1.0 and p2 >= 0.0
+                if p2 > 0:
+                    assert bc == pytest.approx(1.0 - p4 / (3.0 * p2 * p2), rel=1e-12, abs=1e-12)
+                s1, s2, s4, sb = v3[b, 20 + 5 * k:20 + 5 * k + 4]
+                if s2 > 0:
+                    assert sb == pytest.approx(2.0 - s4 / (s2 * s2), rel=1e-12, abs=1e-12)
+            assert 0.0 < v3[b, 40] <= 1.0          # N1fpar in [0, 1]
+        tag = f[len("MC_Bin_AvgValue3"):]
+        hn = np.fromfile(os.path.join(ob, "MC_Bin_NeelHist" + tag), dtype=np.int32)
+        hs = np.fromfile(os.path.join(ob, "MC_Bin_StrpHist" + tag), dtype=np.int32).reshape(41, 4)
+        hv = np.fromfile(os.path.join(ob, "MC_Bin_SMagVecHist" + tag), dtype=np.int32)
+        assert hn.sum() == 2000 and np.all(hs.sum(0) == 2000) and hv.sum() == 2000
+        assert os.path.exists(os.path.join(d, "outData", "NeelHist" + tag[:-4] + ".dat"))
+    # a second run adds to the histogram files
+    H.run_mc(d, 2, precision=H.BATCHED_F64, seed=6)
+    tag = v3f[0][len("MC_Bin_AvgValue3"):]
+    hn = np.fromfile(os.path.join(ob, "MC_Bin_NeelHist" + tag), dtype=np.int32)
+    assert hn.sum() == 4000

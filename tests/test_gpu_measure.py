@@ -124,3 +124,82 @@ def test_sq_cadence_counts_from_the_moments_reset():
     assert nm == 5 and np.all(mom[:, 65] == 1)
     f, cnt = mc.sq_field()
     assert cnt == 4
+
+
+def _qc_states(pbc, n):
+    d = os.path.join(GOLD, "qc_%s_%d" % (pbc, n))
+    rd = lambda k, dt: np.fromfile(os.path.join(d, "%s_AppxtSz(%d).bin" % (k, n)), dtype=dt)
+    nb = rd("nborsList", "<i4").reshape(n, 19)
+    rv = rd("sitesList", "<f8").reshape(n, 2)
+    st = np.stack([rd(k, "<f8").reshape(n, 3) for k in
+                   ["stateNeel0", "stateStrp1", "stateStrp2", "stateStrp3", "stateStrp4"]])
+    return nb, rv, st
+
+
+def _hidx(r):
+    # get_HistIndex(r, 1, 20, 41) with C's round (half away from zero)
+    i = int(math.floor((r + 1.0) * 20.0 + 0.5)) if (r + 1.0) >= 0 else -int(math.floor(-(r + 1.0) * 20.0 + 0.5))
+    return min(max(i, 0), 40)
+
+
+@pytest.mark.parametrize("model,prec", [(H.HEISENBERG, H.BATCHED_F64), (H.HEISENBERG, H.BATCHED_F32),
+                                        (H.ISING, H.BATCHED_F32)])
+def test_quasicrystal_moments_and_histograms_match_oracle(model, prec):
+    _say("qc moments", model, prec)
+    nb, rv, st = _qc_states("no_PBC", 239)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
+    lat = H.Lattice.from_tables(nb, 8, 3, 8, rv, c)
+    pk = lat.set_qc_states(st[0], st[1:], neel_phase=False)
+    R = 4
+    temps = np.geomspace(0.3, 3.0, R)
+    mc = H.MonteCarlo(lat, temps, model=model, precision=prec, n_or=1, n_hb=1, seed=4)
+    rng = np.random.default_rng(8)
+    if model == H.ISING:
+        s = np.zeros((R, lat.N, 3))
+        s[..., 0] = np.where(rng.random((R, lat.N)) < 0.5, -1.0, 1.0)
+        s[0] = st[1]                       # one replica in a reference stripe state
+    else:
+        s = _spins(rng, R, lat.N)
+    mc.set_spins(s)
+    mc.moments(reset=True)
+    mc.qc_histograms(reset=True)
+    n_sw = 3
+    vals = []
+    for _ in range(n_sw):
+        mc.run(1, pt_parity=-1, measure=True)
+        sp = mc.get_spins()
+        dq = 8 * math.pi / lat.Gsz
+        vals.append(cpu_ref.qc_values(lat.tables(), 8, sp, st, rv, lat.Nq, dq, pk))
+    mom, nm = mc.moments()
+    assert nm == n_sw
+    v = np.stack(vals)                     # (sweeps, R, 11)
+    tol = dict(rtol=1e-9, atol=1e-10) if prec == H.BATCHED_F64 else dict(rtol=2e-5, atol=2e-6)
+    P = v[..., 1:5]
+    S1 = np.sqrt((P ** 2).sum(-1))
+    np.testing.assert_allclose(mom[:, 7], np.abs(v[..., 0]).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 8], (v[..., 0] ** 2).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 10], S1.sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 11], (S1 ** 2).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 16], v[..., 5].sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 66:70], np.abs(P).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 70:74], (P ** 2).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 78], np.abs(v[..., 6]).sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 81:85], v[..., 7:11].sum(0), **tol)
+    np.testing.assert_allclose(mom[:, 85:89], (v[..., 7:11] ** 2).sum(0), **tol)
+    assert np.all(mom[:, 13:16] == 0.0)
+    if prec == H.BATCHED_F64:
+        # histograms: bin of every measured value (FP32 values can sit on a bin edge)
+        hn, hs, hv = mc.qc_histograms()
+        for k in range(R):
+            want_n = np.zeros(41, int)
+            want_s = np.zeros((41, 4), int)
+            want_v = np.zeros((41, 41), int)
+            for t in range(n_sw):
+                want_n[_hidx(v[t, k, 0])] += 1
+                for i in range(4):
+                    want_s[_hidx(v[t, k, 1 + i]), i] += 1
+                a = np.abs(v[t, k, 1:5])
+                want_v[_hidx(a[1] - a[3]), _hidx(a[0] - a[2])] += 1
+            np.testing.assert_array_equal(hn[k], want_n)
+            np.testing.assert_array_equal(hs[k], want_s)
+            np.testing.assert_array_equal(hv[k], want_v)

@@ -3,10 +3,12 @@ on the DFT grid and by direct sums, the key points and correlation lengths)
 against derivatives of the oracle's energy, numpy's FFT and ordered states
 whose spectra are known.  CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
 
+import classical_mc_and_smd_for_2d_heisenberg_models_amd as H
 import cpu_ref
 from test_oracle import _random_spins, params
 
@@ -163,3 +165,104 @@ def test_dssf_direct_equals_grid_dssf_on_square_lattice():
     t2[:, 3] = 0.0
     np.testing.assert_allclose(cpu_ref.dssf_direct(traj, rv, q, w, no, mask=mask),
                                cpu_ref.dssf_direct(t2, rv, q, w, no), atol=1e-12)
+
+
+# ------------------------------------------------ quasicrystal observables
+
+def _qc_fixture(pbc, n):
+    d = os.path.join(os.path.dirname(__file__), "golden", "qc_%s_%d" % (pbc, n))
+    rd = lambda k, dt: np.fromfile(os.path.join(d, "%s_AppxtSz(%d).bin" % (k, n)), dtype=dt)
+    nb = rd("nborsList", "<i4").reshape(n, 19)
+    rv = rd("sitesList", "<f8").reshape(n, 2)
+    st = np.stack([rd(k, "<f8").reshape(n, 3) for k in
+                   ["stateNeel0", "stateStrp1", "stateStrp2", "stateStrp3", "stateStrp4"]])
+    return nb, rv, st
+
+
+@pytest.mark.parametrize("pbc,n", [("with_PBC", 41), ("no_PBC", 239)])
+def test_qc_projections_of_the_reference_states(pbc, n):
+    # each +-1 reference state projects on itself with 1; a saturated stripe
+    # projection zeroes the other three (the cut of get_qctStrpPrjt)
+    nb, rv, st = _qc_fixture(pbc, n)
+    v = cpu_ref.qc_values(nb, 8, st, st, rv, n, 1.0, np.zeros((8, 2), np.int32))
+    assert v[0, 0] == pytest.approx(1.0, abs=1e-14)
+    for i in range(4):
+        assert v[1 + i, 1 + i] == pytest.approx(1.0, abs=1e-14)
+        assert np.all(np.delete(v[1 + i, 1:5], i) == 0.0)
+    # cross projections of unsaturated states are plain overlaps / N
+    assert v[0, 1] == pytest.approx(np.sum(st[0] * st[1]) / n, abs=1e-14)
+
+
+def _star():
+    # site 0 with eight clock-ordered shell-1 neighbours 1..8; each of them
+    # has site 0 in slot 0
+    n = 9
+    nb = -np.ones((n, 19), np.int32)
+    nb[0, :8] = np.arange(1, 9)
+    nb[1:, 0] = 0
+    rv = np.zeros((n, 2))
+    return nb, rv
+
+
+def test_qc_stripe_vector_and_frustration_closed_forms():
+    nb, rv = _star()
+    s = np.zeros((9, 3))
+    s[0, 0] = 1.0
+    # slots 0, 1, 4, 5 up, slots 2, 3, 6, 7 down: site 0 gives (rx, ry) = (1, 0);
+    # each outer site gives (+-1, -+1) clipped to length 1, cancelling in pairs
+    sgn = np.array([1, 1, -1, -1, 1, 1, -1, -1], float)
+    s[1:, 0] = sgn
+    v = cpu_ref.qc_values(nb, 8, s, np.zeros((5, 9, 3)), rv, 9, 1.0, np.zeros((8, 2), np.int32))
+    assert v[0, 5] == pytest.approx(1.0 / 9.0, abs=1e-15)
+    # N1fpar: site 0 sees a zero average, outer sites (1 - s0.s_j) / 2 each
+    assert v[0, 6] == pytest.approx((0.5 + np.sum(0.5 * (1 - sgn))) / 9.0, abs=1e-15)
+    # staggered star: every bond antiparallel -> N1fpar = 1; ferro -> 0
+    s[1:, 0] = -1.0
+    v = cpu_ref.qc_values(nb, 8, s, np.zeros((5, 9, 3)), rv, 9, 1.0, np.zeros((8, 2), np.int32))
+    assert v[0, 6] == pytest.approx(1.0, abs=1e-15)
+    s[1:, 0] = 1.0
+    v = cpu_ref.qc_values(nb, 8, s, np.zeros((5, 9, 3)), rv, 9, 1.0, np.zeros((8, 2), np.int32))
+    assert v[0, 6] == pytest.approx(0.0, abs=1e-15)
+    # ferro star: site 0 has vec1 = vec2 = 0; each outer site has its one
+    # neighbour in slot 0, (rx, ry) = (1, -1) clipped to (1, -1)/sqrt 2:
+    # |sum| / 9 = 8 / 9
+    assert v[0, 5] == pytest.approx(8.0 / 9.0, abs=1e-15)
+
+
+def test_qc_hotspots_and_amplitudes_of_a_plane_wave():
+    # periodic 14 x 14 grid, dq = 2 pi / 14: a spiral at q0 = dq (4, 2) has
+    # |S(q)|^2 only at +-q0 (discrete orthogonality), so the arc search
+    # (2 < |q| < 5, 0 < arg q < pi/4) finds q0 and the eight images follow;
+    # SqPeaks[0] = sqrt(0.5 / Nq (N^2/6 + N^2/6)), the other three vanish
+    Lg = 14
+    x, y = np.meshgrid(np.arange(Lg), np.arange(Lg), indexing="xy")
+    rv = np.stack([x.ravel(), y.ravel()], 1).astype(float)
+    n = Lg * Lg
+    dq = 2 * math.pi / Lg
+    th = dq * (4 * rv[:, 0] + 2 * rv[:, 1])
+    s = np.stack([np.cos(th), np.sin(th), np.zeros(n)], 1)
+    pk = cpu_ref.qc_peaks(s, rv, Lg, dq)
+    want = [(4, 2), (-4, -2), (2, 4), (-2, -4), (-4, 2), (2, -4), (4, -2), (-2, 4)]
+    assert [tuple(p) for p in pk.tolist()] == want
+    nb = -np.ones((n, 19), np.int32)
+    v = cpu_ref.qc_values(nb, 8, s, np.zeros((5, n, 3)), rv, n, dq, pk)
+    assert v[0, 7] == pytest.approx(math.sqrt(n * n / (6.0 * n)), rel=1e-12)
+    np.testing.assert_allclose(v[0, 8:11], 0.0, atol=1e-10)
+
+
+def test_qc_hotspots_of_the_library_match_the_oracle():
+    # host-side search of the library (hsmd_lattice_set_qc_states) against
+    # the oracle on both fixtures and both reference choices; the 41-site
+    # fixture has no grid point in the arc (hotspots at q = 0)
+    for pbc, n in [("with_PBC", 41), ("no_PBC", 239)]:
+        nb, rv, st = _qc_fixture(pbc, n)
+        lat = H.Lattice.from_tables(nb, 8, 3, 8, rv, H.Couplings(J1=1.0, J2=0.4, JX=0.1))
+        dq = 8 * math.pi / lat.Gsz
+        for neel in (True, False):
+            got = lat.set_qc_states(st[0], st[1:], neel_phase=neel)
+            ref = cpu_ref.qc_peaks(st[0] if neel else st[1], rv, lat.L, dq)
+            np.testing.assert_array_equal(got, ref)
+            if n == 41:
+                assert np.all(got == 0)
+            else:
+                assert np.any(got != 0)

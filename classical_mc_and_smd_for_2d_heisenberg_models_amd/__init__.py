@@ -107,6 +107,7 @@ SIGNATURES = {
     "hsmd_mc_set_threads": (_I, [_P, _I]),
     "hsmd_mc_autotune": (_I, [_P, _I, _D, ctypes.POINTER(MCSpec), _I, ctypes.POINTER(MCTuning)]),
     "hsmd_smd_set_shape": (_I, [_P, _I, _I]),
+    "hsmd_smd_set_samples_from_mc": (_I, [_P, _P, _I32]),
     "hsmd_smd_autotune": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D, _I,
                                ctypes.POINTER(SMDTuning)]),
     "hsmd_lattice_destroy": (None, [_P]),
@@ -464,6 +465,14 @@ class SpinDynamics:
         2 (LDS, per-workgroup classes), 3 (global memory); gemm 0 (128 x 128
         contraction tile), 1 (64 x 64) or 2 (128 x 256)."""
         _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
+
+    def set_samples_from_mc(self, mc, slots):
+        """Sample s starts from the configuration at temperature slot
+        slots[s] of the MonteCarlo engine mc, copied on the device."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        if sl.shape != (self.S,):
+            raise ValueError("one slot per sample")
+        _check(lib().hsmd_smd_set_samples_from_mc(self._h, mc._h, _ip(sl)))
 
     def set_samples(self, spins_list):
         flat = np.concatenate([np.asarray(s, dtype=np.float64).reshape(-1) for s in spins_list])

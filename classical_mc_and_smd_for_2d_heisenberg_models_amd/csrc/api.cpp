@@ -53,6 +53,8 @@ void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out);
 void smd_energies(SMDEngine &e, double *E);
 void smd_destroy(SMDEngine *e);
 void smd_set_shape(SMDEngine &e, int form, int gemm);
+McHandoff mc_handoff(MCEngine &e);
+void smd_set_samples_from_mc(SMDEngine &e, const McHandoff &h, const int32_t *slots);
 void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
                   const hsmd_smd_spec &sp, const double *window, int n_steps, hsmd_smd_tuning *out);
 
@@ -381,6 +383,13 @@ void hsmd_mc_destroy(hsmd_mc *mc) {
   if (!mc) return;
   mc_destroy(mc->e);
   delete mc;
+}
+
+hsmd_status hsmd_smd_set_samples_from_mc(hsmd_smd *smd, hsmd_mc *mc, const int32_t *slots) {
+  return guarded([&] {
+    REQUIRE(smd && mc && slots, "null argument");
+    smd_set_samples_from_mc(*smd->e, mc_handoff(*mc->e), slots);
+  });
 }
 
 hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm) {

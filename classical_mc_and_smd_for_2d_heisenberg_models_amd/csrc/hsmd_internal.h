@@ -106,6 +106,16 @@ Lattice *lattice_from_tables(int n, int zn1, int zn2, int znx, const int32_t *nb
                              const double *rvec, const hsmd_couplings &c);
 void derive_flags(Lattice &lat);
 
+// MC -> SMD hand-off on the device: where an MC engine keeps its
+// configurations (mc.hip mc_handoff, read by smd.hip)
+struct McHandoff {
+  const void *spins;          // per configuration [3][N+1] (int8: [N+1])
+  int type;                   // 0 int8 (Ising), 1 FP32, 2 FP64
+  int R;
+  const int32_t *cfg_of_slot; // device, R
+  const Lattice *lat;         // labels: lat->perm (label -> site)
+};
+
 // qc.cpp: quasicrystal reference states and hotspots
 void lattice_set_qc(Lattice &L, const double *neel, const double *strp, bool neel_phase);
 

@@ -1344,6 +1344,17 @@ void mc_qc_histograms(MCEngine &e, int32_t *out, int reset) {
   if (reset) e.qc_hist.zero();
 }
 
+McHandoff mc_handoff(MCEngine &e) {
+  if (!e.spins_set) throw Error(HSMD_ERR_STATE, "the MC engine has no spins yet");
+  McHandoff h;
+  h.spins = e.spins.p;
+  h.type = e.ising ? 0 : (e.prec == HSMD_BATCHED_F32 ? 1 : 2);
+  h.R = e.R;
+  h.cfg_of_slot = e.cfg_of_slot.p;
+  h.lat = &e.lat;
+  return h;
+}
+
 void mc_slot_map(MCEngine &e, int32_t *m) {
   HIP_CHECK(hipMemcpy(m, e.cfg_of_slot.p, e.R * sizeof(int32_t), hipMemcpyDeviceToHost));
 }

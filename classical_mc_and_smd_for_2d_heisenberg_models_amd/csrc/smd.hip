@@ -499,6 +499,94 @@ void smd_set_samples(SMDEngine &e, const double *spins) {
   e.set = true;
 }
 
+// Sample s <- the configuration at temperature slot slots[s] of an MC
+// engine, copied on the device (MC labels -> SMD labels, MC precision -> SMD
+// precision).  map[p] = MC label of the site at SMD label p.
+template <typename TD, typename TS>
+__global__ void handoff_kernel(const TS *src, int comps, int N, const int32_t *cfg_of_slot,
+                               const int32_t *slots, const int32_t *map, const SampleDev *samples,
+                               TD *dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (p >= N) return;
+  const int c = cfg_of_slot[slots[s]];
+  const int q = map[p];
+  const TS *g = src + (size_t)c * comps * (N + 1);
+  TD *o = dst + samples[s].spin_off;
+  for (int d = 0; d < 3; d++) o[(size_t)d * (N + 1) + p] = d < comps ? (TD)g[(size_t)d * (N + 1) + q] : TD(0);
+}
+
+void smd_set_samples_from_mc(SMDEngine &e, const McHandoff &h, const int32_t *slots) {
+  const Lattice &ML = *h.lat;
+  const uint64_t sig = lattice_signature(ML);
+  for (int s = 0; s < e.S; s++) {
+    if (slots[s] < 0 || slots[s] >= h.R) throw Error(HSMD_ERR_ARG, "slot index out of range");
+    const Lattice &L = e.lats[e.hs[s].li];
+    if (L.N != ML.N || lattice_signature(L) != sig)
+      throw Error(HSMD_ERR_ARG, "every sample must be on the MC engine's lattice");
+  }
+  const int N = ML.N;
+  std::vector<int32_t> mcinv(N);
+  for (int p = 0; p < N; p++) mcinv[ML.perm[p]] = p;
+  // one label map per lattice of the batch (equal lattices given as
+  // separate entries are handled separately)
+  DevBuf<int32_t> dmap;
+  std::vector<int> done(e.lats.size(), 0);
+  for (int s = 0; s < e.S; s++) {
+    const int li = e.hs[s].li;
+    if (done[li]) continue;
+    done[li] = 1;
+    const Lattice &L = e.lats[li];
+    std::vector<int32_t> map(N);
+    for (int p = 0; p < N; p++) map[p] = mcinv[L.perm[p]];
+    dmap.upload(map.data(), N);
+    std::vector<int32_t> ids;
+    for (int t = 0; t < e.S; t++)
+      if (e.hs[t].li == li) ids.push_back(t);
+    // one launch per lattice: the samples of this lattice, by sample index
+    std::vector<int32_t> sl(ids.size());
+    std::vector<SampleDev> sd(ids.size());
+    for (size_t k = 0; k < ids.size(); k++) { sl[k] = slots[ids[k]]; sd[k] = e.hs[ids[k]]; }
+    DevBuf<int32_t> dsl;
+    DevBuf<SampleDev> dsd;
+    dsl.upload(sl.data(), sl.size());
+    dsd.upload(sd.data(), sd.size());
+    const dim3 grid((N + 255) / 256, (unsigned)ids.size());
+    const int comps = h.type == 0 ? 1 : 3;
+#define HSMD_HANDOFF(TD, TS)                                                                     \
+  hipLaunchKernelGGL((handoff_kernel<TD, TS>), grid, dim3(256), 0, 0, (const TS *)h.spins, comps, N, \
+                     h.cfg_of_slot, dsl.p, dmap.p, dsd.p, (TD *)e.spins.p)
+    if (e.f32) {
+      if (h.type == 0) HSMD_HANDOFF(float, int8_t);
+      else if (h.type == 1) HSMD_HANDOFF(float, float);
+      else HSMD_HANDOFF(float, double);
+    } else {
+      if (h.type == 0) HSMD_HANDOFF(double, int8_t);
+      else if (h.type == 1) HSMD_HANDOFF(double, float);
+      else HSMD_HANDOFF(double, double);
+    }
+#undef HSMD_HANDOFF
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+  }
+  HIP_CHECK(hipMemcpy(e.s0.p, e.spins.p, e.spins.n, hipMemcpyDeviceToDevice));
+  e.t_next = 0;
+  e.chunk_base = 0;
+  e.chunk_fill = 0;
+  e.dssf_done = false;
+  e.qss.zero();
+  if (e.spec.dssf) e.ybuf.zero();
+  if (e.f32)
+    hipLaunchKernelGGL(smd_energy_kernel<float>, dim3(e.S), dim3(kSmdThreads), 0, 0, e.dlats.p,
+                       e.dsamples.p, (const float *)e.spins.p, e.eref.p);
+  else
+    hipLaunchKernelGGL(smd_energy_kernel<double>, dim3(e.S), dim3(kSmdThreads), 0, 0, e.dlats.p,
+                       e.dsamples.p, (const double *)e.spins.p, e.eref.p);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  e.set = true;
+}
+
 void smd_get_samples(SMDEngine &e, double *spins) {
   const size_t total = e.spins.n / (e.f32 ? sizeof(float) : sizeof(double));
   std::vector<double> h64;

@@ -314,6 +314,13 @@ hsmd_status hsmd_smd_autotune(const hsmd_lattice *const *lattices, int n_lattice
                               int n_steps, hsmd_smd_tuning *out);
 /* Spins of every sample, concatenated in sample order (sum_s n_s x 3 f64). */
 hsmd_status hsmd_smd_set_samples(hsmd_smd *smd, const double *spins);
+/* MC -> SMD hand-off on the device: sample s starts from the configuration
+ * at temperature slot slots[s] of mc (every sample must be on the MC
+ * engine's lattice).  Replaces the round trip through MC_Bin_SpinField
+ * (written at CF/mc_hm_code.cpp:5167-5177, read at CF/smd_code.cpp:3079-3130)
+ * for in-process pipelines; same effect as hsmd_smd_set_samples with those
+ * spins. */
+hsmd_status hsmd_smd_set_samples_from_mc(hsmd_smd *smd, hsmd_mc *mc, const int32_t *slots);
 hsmd_status hsmd_smd_get_samples(hsmd_smd *smd, double *spins);
 /* Advances every sample n_steps RK4 steps (each with its energy correction);
  * replaces the time loop of get_dynSpectrum (CF/subrouts.cpp:11734-11927).

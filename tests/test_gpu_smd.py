@@ -245,3 +245,54 @@ def test_poisson_disk_graph_with_vacancies_matches_oracle():
     for k in range(3):
         s_ref = m.smd_run(samples[k], 0.01, T, gauss_seidel=False, traj=False)[0]
         assert np.abs(out[k] - s_ref).max() < 1e-10, k
+
+
+@pytest.mark.parametrize("case", ["square_f32", "graph_f64", "ising_to_f32"])
+def test_mc_to_smd_handoff_on_device_equals_host_round_trip(case):
+    # the device hand-off (MC colour labels -> SMD labels, with precision
+    # conversion) starts the dynamics exactly where set_samples with the MC
+    # engine's spins does: same samples, same trajectory, same DSSF
+    _say("handoff", case)
+    c = H.Couplings(J1=1.0, J2=0.5)
+    if case == "graph_f64":
+        pts = H.poisson_disk(16.0, 1.0, seed=2)
+        lat = H.Lattice.from_points(pts, 16.0, (1.15, 1.45, 1.75), H.Couplings(J1=1.0, J2=0.4, JX=0.1))
+        mprec, sprec, model = H.BATCHED_F64, H.BATCHED_F64, H.HEISENBERG
+    elif case == "square_f32":
+        lat = H.Lattice.generate("square", 16, c)
+        mprec, sprec, model = H.BATCHED_F32, H.BATCHED_F32, H.HEISENBERG
+    else:
+        lat = H.Lattice.generate("square", 16, c)
+        mprec, sprec, model = H.BATCHED_F32, H.BATCHED_F32, H.ISING
+    R = 4
+    mc = H.MonteCarlo(lat, np.geomspace(0.3, 2.0, R), model=model, precision=mprec, seed=6)
+    rng = np.random.default_rng(4)
+    if model == H.ISING:
+        s = np.zeros((R, lat.N, 3))
+        s[..., 0] = np.where(rng.random((R, lat.N)) < 0.5, -1.0, 1.0)
+    else:
+        v = rng.normal(size=(R, lat.N, 3))
+        s = v / np.linalg.norm(v, axis=-1, keepdims=True)
+    mc.set_spins(s)
+    mc.run(12, pt_parity=2)
+    slots = np.array([3, 0, 3, 1, 2], np.int32)
+    T = 64
+    a = H.SpinDynamics([lat], np.zeros(5, np.int32), 0.01, T, n_omega=32, precision=sprec)
+    b = H.SpinDynamics([lat], np.zeros(5, np.int32), 0.01, T, n_omega=32, precision=sprec)
+    a.set_samples_from_mc(mc, slots)
+    sp = mc.get_spins()
+    b.set_samples([sp[k] for k in slots])
+    for x, y in zip(a.get_samples(), b.get_samples()):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a.energies(), b.energies())
+    a.step(T - 1)
+    b.step(T - 1)
+    for x, y in zip(a.get_samples(), b.get_samples()):
+        np.testing.assert_array_equal(x, y)
+    if case != "graph_f64":
+        np.testing.assert_array_equal(a.dssf(0)[0], b.dssf(0)[0])
+    # a sample on another lattice is refused
+    other = H.Lattice.generate("square", 8, c)
+    e = H.SpinDynamics([lat, other], np.array([0, 1], np.int32), 0.01, T, n_omega=32, precision=sprec)
+    with pytest.raises(H.HSMDError):
+        e.set_samples_from_mc(mc, np.array([0, 1], np.int32))

@@ -143,6 +143,8 @@ SIGNATURES = {
                                ctypes.c_uint32, _I, _U32]),
     "hsmd_debug_exp": (_I, [_I, _D, _D]),
     "hsmd_debug_mc_sweeps": (_I, [_P, _I]),
+    "hsmd_bin_statistics": (_I, [_D, _I, _I, _I, ctypes.c_uint32, _D, _D]),
+    "hsmd_debug_mt19937": (_I, [ctypes.c_uint32, _I, _U32]),
 }
 
 _lib = None
@@ -549,6 +551,23 @@ def debug_philox(seed, step, inst, stream, n):
     out = np.zeros(4 * n, np.uint32)
     _check(lib().hsmd_debug_philox(seed, step, inst, stream, n, out.ctypes.data_as(_U32)))
     return out.reshape(n, 4)
+
+
+def bin_statistics(data, bootstrap=True, seed=0):
+    """Mean and deviation over bins (data: n_bins x n_values), as the
+    reference's MC_Stat_Results (get_statVectors)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    nb, nv = d.shape
+    m = np.zeros(nv)
+    s = np.zeros(nv)
+    _check(lib().hsmd_bin_statistics(_dp(d), nb, nv, int(bootstrap), int(seed) & 0xffffffff, _dp(m), _dp(s)))
+    return m, s
+
+
+def debug_mt19937(seed, n):
+    out = np.zeros(n, np.uint32)
+    _check(lib().hsmd_debug_mt19937(int(seed) & 0xffffffff, n, out.ctypes.data_as(_U32)))
+    return out
 
 
 def debug_exp(x):

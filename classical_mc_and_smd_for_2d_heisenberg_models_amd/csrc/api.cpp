@@ -495,6 +495,21 @@ hsmd_status hsmd_debug_mc_sweeps(hsmd_mc *mc, int n) {
   });
 }
 
+hsmd_status hsmd_bin_statistics(const double *data, int n_bins, int n_values, int bootstrap,
+                                uint32_t seed, double *mean, double *sdev) {
+  return guarded([&] {
+    REQUIRE(data && mean && sdev, "null argument");
+    bin_statistics(data, n_bins, n_values, bootstrap != 0, seed, mean, sdev);
+  });
+}
+
+hsmd_status hsmd_debug_mt19937(uint32_t seed, int n, uint32_t *out) {
+  return guarded([&] {
+    REQUIRE(out && n >= 0, "bad argument");
+    debug_mt19937(seed, n, out);
+  });
+}
+
 hsmd_status hsmd_debug_exp(int n, const double *x, double *y) {
   return guarded([&] {
     REQUIRE(x && y && n >= 0, "bad argument");

@@ -630,6 +630,15 @@ void run_mc(const std::string &dir, int npt, int precision, uint64_t seed, bool 
     }
   }
   write_bin(dir + "/outBin/Temp_List" + s.label + ".bin", T.data(), T.size(), false);
+  // statistics of the bin averages (CF/mc_hm_code.cpp:6430-7239)
+  if (NBins >= 2) {
+    std::vector<std::string> tags(npt);
+    for (int k = 0; k < npt; k++) tags[k] = t_tag(T[k], s.label);
+    const std::string info = s.outTag == "Htag" ? "  Extn. Magnetic Field"
+                             : (s.outTag == "J2tag" ? "  Coupling strength J2" : "  Coupling strength J1");
+    const double lv = s.outTag == "Htag" ? s.h : (s.outTag == "J2tag" ? s.J2 : s.J1);
+    write_mc_stats(dir, s.label, lv, info, T, tags, (uint32_t)seed);
+  }
 }
 
 // Full SMD run over every recorded sample at temperature TpVec[tpIndex]

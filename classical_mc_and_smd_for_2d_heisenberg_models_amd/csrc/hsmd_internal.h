@@ -116,6 +116,14 @@ struct McHandoff {
   const Lattice *lat;         // labels: lat->perm (label -> site)
 };
 
+// stats.cpp: bin statistics and the MC_Stat_Results / MC_Collect files
+void bin_statistics(const double *data, int n_bins, int n_values, bool bootstrap, uint32_t seed,
+                    double *mean, double *sdev);
+void debug_mt19937(uint32_t seed, int n, uint32_t *out);
+void write_mc_stats(const std::string &dir, const std::string &label, double label_value,
+                    const std::string &info, const std::vector<double> &T,
+                    const std::vector<std::string> &tags, uint32_t seed);
+
 // qc.cpp: quasicrystal reference states and hotspots
 void lattice_set_qc(Lattice &L, const double *neel, const double *strp, bool neel_phase);
 

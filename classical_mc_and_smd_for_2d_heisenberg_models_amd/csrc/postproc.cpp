@@ -293,7 +293,7 @@ void write_smd_spectra(const std::string &dir, const Lattice &L, const double *d
 // [41 imag][41 real]) added to the counts already in the binary files
 // MC_Bin_NeelHist / StrpHist / SMagVecHist<tag>.bin (a file of another size
 // is an error, as in the reference), written back, and the normalised
-// histograms as outData/NeelHist / StrpHist / SMagVecHist<tag>.dat.
+// histograms as outData/MC_Results/NeelHist / StrpHist / SMagVecHist<tag>.dat.
 void write_qc_histograms(const std::string &dir, const int32_t *h, const std::string &tag) {
   constexpr int H1 = 41;
   const double box = 2.0 / (H1 - 1);
@@ -328,7 +328,9 @@ void write_qc_histograms(const std::string &dir, const int32_t *h, const std::st
     for (int k = 0; k < 4; k++) n4[k] += strp[4 * i + k];
   }
   for (int v : vec) nv += v;
-  const std::string base = dir + "/outData/";
+  const std::string base = dir + "/outData/MC_Results/";
+  make_dir(dir + "/outData/");
+  make_dir(base);
   std::ofstream o1(base + "NeelHist" + tag + ".dat"), o2(base + "StrpHist" + tag + ".dat");
   if (!o1 || !o2) throw Error(HSMD_ERR_IO, "cannot write histogram files in " + base);
   for (int i = 0; i < H1; i++) {

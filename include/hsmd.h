@@ -363,7 +363,21 @@ hsmd_status hsmd_run_mc(const char *workdir, int n_replicas, int precision, uint
  * writes the sample-averaged outBin/MC_Bin_SpecData_T(..)<label>.bin. */
 hsmd_status hsmd_run_smd(const char *workdir, int precision, int strict);
 
+/* --------------------------------------------------------- statistics */
+
+/* Mean and standard deviation over bins of n_values bin averages
+ * (data: n_bins x n_values, bin-major) -- get_statVectors
+ * (CF/aux_code.cpp:1050-1187).  bootstrap = 1: deviation of the means of
+ * 1000 resamples with replacement drawn with MT19937 seeded by `seed` and
+ * GSL's uniform_int rule (the reference's USE_GSL branch); 0: the sample
+ * standard deviation. */
+hsmd_status hsmd_bin_statistics(const double *data, int n_bins, int n_values, int bootstrap,
+                                uint32_t seed, double *mean, double *sdev);
+
 /* --------------------------------------------------------- self checks */
+
+/* First n outputs of the MT19937 generator the bootstrap uses. */
+hsmd_status hsmd_debug_mt19937(uint32_t seed, int n, uint32_t *out);
 
 /* Writes n Philox4x32-10 blocks for counters (site = i, step, inst, stream)
  * computed on the device (4n uint32), for the bit-exact stream check. */

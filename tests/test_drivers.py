@@ -78,6 +78,20 @@ def test_mc_then_smd_pipeline_files_and_dssf(tmp_path):
         sf = np.fromfile(os.path.join(d, "outBin", "MC_Bin_SpinField" + tag)).reshape(-1, N, 3)
         assert sf.shape[0] == 2000 // 500
         np.testing.assert_allclose(np.linalg.norm(sf, axis=-1), 1.0, atol=1e-12)
+    # MC_Stat_Results: one row per temperature, T and the label value, then
+    # (mean, bootstrap deviation) of every Values1 entry over the bins
+    st = np.loadtxt(os.path.join(d, "outData", "MC_Stat_Results1%s.dat" % label))
+    assert st.shape == (4, 2 + 2 * 23)
+    np.testing.assert_allclose(st[:, 0], T, rtol=1e-12)
+    assert np.all(st[:, 1] == 0.6)
+    for k, t in enumerate(T):
+        v1 = np.fromfile(os.path.join(d, "outBin", "MC_Bin_AvgValue1_T(%.5f)%s.bin" % (t, label)))
+        m, sd = H.bin_statistics(v1.reshape(-1, 23), bootstrap=True, seed=3 + k)
+        np.testing.assert_allclose(st[k, 2::2], m, rtol=1e-11, atol=1e-300)
+        np.testing.assert_allclose(st[k, 3::2], sd, rtol=1e-11, atol=1e-300)
+    fx = np.loadtxt(os.path.join(d, "outData", "MC_Collect", "MC_Stat_FixTemp1(%.5f).dat" % T[2]),
+                    ndmin=2)
+    np.testing.assert_allclose(fx[0], st[2], rtol=1e-12)
     nb = np.fromfile(os.path.join(d, "outBin", "nborsList.bin"), dtype=np.int32).reshape(N, 12)
     ref_nb, _, gm, zn = cpu_ref.lattice("square", 6)
     np.testing.assert_array_equal(nb, ref_nb)
@@ -208,7 +222,7 @@ def test_quasicrystal_run_writes_values3_and_histograms(tmp_path):
         hs = np.fromfile(os.path.join(ob, "MC_Bin_StrpHist" + tag), dtype=np.int32).reshape(41, 4)
         hv = np.fromfile(os.path.join(ob, "MC_Bin_SMagVecHist" + tag), dtype=np.int32)
         assert hn.sum() == 2000 and np.all(hs.sum(0) == 2000) and hv.sum() == 2000
-        assert os.path.exists(os.path.join(d, "outData", "NeelHist" + tag[:-4] + ".dat"))
+        assert os.path.exists(os.path.join(d, "outData", "MC_Results", "NeelHist" + tag[:-4] + ".dat"))
     # a second run adds to the histogram files
     H.run_mc(d, 2, precision=H.BATCHED_F64, seed=6)
     tag = v3f[0][len("MC_Bin_AvgValue3"):]

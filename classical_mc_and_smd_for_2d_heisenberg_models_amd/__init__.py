@@ -465,7 +465,8 @@ class SpinDynamics:
     def set_shape(self, form, gemm):
         """Launch shape of the batched forms: form 1 (LDS, one slot class),
         2 (LDS, per-workgroup classes), 3 (global memory); gemm 0 (128 x 128
-        contraction tile), 1 (64 x 64) or 2 (128 x 256)."""
+        contraction tile), 1 (64 x 64), 2 (128 x 256) or 3 (128 x 128, two
+        LDS stages)."""
         _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
 
     def set_samples_from_mc(self, mc, slots):

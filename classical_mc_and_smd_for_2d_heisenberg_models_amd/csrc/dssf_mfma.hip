@@ -294,15 +294,137 @@ void time_gemm_mfma_q256_kernel(const GemmSample *samples, const float *__restri
   time_gemm_q_body<256>(samples, A, Bth, Btl, ldt, k0, K, ldb, Y, Ah, Al, Bh, Bl);
 }
 
-// tj: 128 or 256 columns per workgroup tile
+// Double-buffered form of the 128 x 128 tile: two LDS stages (80 KB, two
+// workgroups per CU).  Step k's MFMAs read one stage while the same waves
+// convert and store step k+1's tiles into the other, so the FP16 split and
+// the LDS writes issue between matrix instructions, and one barrier per step
+// separates the stages.
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void time_gemm_mfma_db_kernel(const GemmSample *samples, const float *__restrict__ A,
+                              const _Float16 *__restrict__ Bth, const _Float16 *__restrict__ Btl,
+                              int ldt, int k0, int K, int ldb, float *Y) {
+  constexpr int TJ = 128, NB = TJ / 32, BK = WK * TJ / QT;
+  __shared__ __align__(16) _Float16 Ah[2][QM * WKP], Al[2][QM * WKP];
+  __shared__ __align__(16) _Float16 Bh[2][TJ * WKP], Bl[2][TJ * WKP];
+  const GemmSample sd = samples[blockIdx.z];
+  const int M = sd.M, lda = sd.lda;
+  const int nj = ldb / TJ;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * TJ;
+  if (m0 >= M) return;
+  const float *Ab = A + sd.a_off;
+  float *Yb = Y + sd.y_off;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave >> 1) * 64, wj = (wave & 1) * (TJ / 2);
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  const int am = tid % QM;
+  const int kh = __builtin_amdgcn_readfirstlane(tid / QM);
+  const bool arow = m0 + am < M;
+  const __amdgpu_buffer_rsrc_t arsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)Ab, (short)0, 0x7fffffff, 0x00020000);
+  const int acol = 4 * min(m0 + am, lda - 1);
+  const int lda4 = 4 * lda;
+  const int bc = tid % TJ, kb = (tid / TJ) * BK;
+  const size_t bsrc = (size_t)(j0 + bc) * ldt + (size_t)(k0 + kb);
+  float4_t acc[4][NB];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  float ra[16];
+  uint4 rbh0, rbh1, rbl0, rbl1;
+  auto fetch = [&](int kk) {
+    if (kk + WK <= K) {
+      const int r0 = (kk + 16 * kh) * lda4;
+#pragma unroll
+      for (int e = 0; e < 16; e++)
+        ra[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(arsrc, acol, r0 + e * lda4, 0));
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int ka = kk + 16 * kh + e;
+        ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
+      }
+    }
+    const uint4 *h = (const uint4 *)(Bth + bsrc + kk);
+    const uint4 *l = (const uint4 *)(Btl + bsrc + kk);
+    rbh0 = h[0]; rbh1 = h[1];
+    rbl0 = l[0]; rbl1 = l[1];
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      half8_t hv, lv;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float x = ra[8 * g + e];
+        const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) & 0xffffe000u);
+        hv[e] = (_Float16)h;
+        lv[e] = (_Float16)(x - h);
+      }
+      *(half8_t *)&Ah[buf][am * WKP + 16 * kh + 8 * g] = hv;
+      *(half8_t *)&Al[buf][am * WKP + 16 * kh + 8 * g] = lv;
+    }
+    uint4 *dh = (uint4 *)&Bh[buf][bc * WKP + kb];
+    uint4 *dl = (uint4 *)&Bl[buf][bc * WKP + kb];
+    dh[0] = rbh0; dh[1] = rbh1;
+    dl[0] = rbl0; dl[1] = rbl1;
+  };
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  int cur = 0;
+  for (int kk = 0; kk < K; kk += WK) {
+    const bool more = kk + WK < K;
+    if (more) fetch(kk + WK);
+    half8_t ah[4], al[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int row = wm + a * 16 + li;
+      ah[a] = *(const half8_t *)&Ah[cur][row * WKP + lk];
+      al[a] = *(const half8_t *)&Al[cur][row * WKP + lk];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int col = wj + b * 16 + li;
+      const half8_t bh = *(const half8_t *)&Bh[cur][col * WKP + lk];
+      const half8_t bl = *(const half8_t *)&Bl[cur][col * WKP + lk];
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
+      }
+    }
+    if (more) stage(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int j = j0 + wj + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
+        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+      }
+    }
+}
+
+// tj: 128 or 256 columns per workgroup tile; db: the double-buffered 128 tile
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
-                                int ldb, float *Y, hipStream_t st, int tj) {
+                                int ldb, float *Y, hipStream_t st, int tj, bool db) {
   if (tj != 128 && tj != 256) throw Error(HSMD_ERR_ARG, "contraction tile must be 128 or 256 columns");
   if (ldb % tj != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of the tile width");
   const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
   dim3 grid(nm * (ldb / tj), 1, S);
-  if (tj == 128)
+  if (db && tj == 128)
+    hipLaunchKernelGGL(time_gemm_mfma_db_kernel, grid, dim3(QT), 0, st,
+                       (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
+  else if (tj == 128)
     hipLaunchKernelGGL(time_gemm_mfma_q_kernel, grid, dim3(QT), 0, st,
                        (const GemmSample *)samples_dev, A, Bth, Btl, ldt, k0, K, ldb, Y);
   else

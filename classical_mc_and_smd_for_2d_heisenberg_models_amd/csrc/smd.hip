@@ -160,7 +160,7 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
-                                int ldb, float *Y, hipStream_t st, int tj);
+                                int ldb, float *Y, hipStream_t st, int tj, bool db);
 
 struct GemmSampleHost {
   int M;
@@ -617,7 +617,7 @@ static void flush_chunk(SMDEngine &e) {
   if (e.f32 && e.ldt > 0 && e.gemm != 1)
     launch_time_gemm_mfma_wide(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw_hi.p,
                                e.tw_lo.p, e.ldt, e.chunk_base, e.chunk_fill, ldb,
-                               (float *)e.ybuf.p, 0, e.gemm == 2 ? 256 : 128);
+                               (float *)e.ybuf.p, 0, e.gemm == 2 ? 256 : 128, e.gemm == 3);
   else if (e.f32)
     launch_time_gemm_mfma(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p,
                           (const float *)e.twid.p, ldb, e.chunk_base, e.chunk_fill,
@@ -837,13 +837,13 @@ static int current_form(const SMDEngine &e) {
 // batches mixing two classes); 3 = global-memory form (one workgroup per
 // sample, stages in HBM / Infinity Cache).  gemm: 0 = 128 x 128 contraction
 // tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64,
-// 2 = 128 x 256 with pre-split twiddles.
+// 2 = 128 x 256 with pre-split twiddles, 3 = 128 x 128 with two LDS stages.
 // Per-site arithmetic is the same in every shape; FP64 reductions may group
 // terms differently.
 void smd_set_shape(SMDEngine &e, int form, int gemm) {
   if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
   if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
-  if (gemm < 0 || gemm > 2) throw Error(HSMD_ERR_ARG, "gemm tile must be 0, 1 or 2");
+  if (gemm < 0 || gemm > 3) throw Error(HSMD_ERR_ARG, "gemm tile must be 0, 1, 2 or 3");
   if ((form == 1 || form == 2) && !e.lds_ok)
     throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
   if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
@@ -949,7 +949,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
     }
     if (sp.dssf && e->f32) {
       std::vector<int> tiles = {1};
-      if (e->ldt > 0) tiles = {0, 2, 1};
+      if (e->ldt > 0) tiles = {0, 3, 2, 1};
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);

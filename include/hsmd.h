@@ -286,7 +286,7 @@ hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
 /* Result of hsmd_smd_autotune. */
 typedef struct hsmd_smd_tuning {
   int32_t form;              /* for hsmd_smd_set_shape: 1, 2 or 3 */
-  int32_t gemm;              /* for hsmd_smd_set_shape: 0, 1 or 2 */
+  int32_t gemm;              /* for hsmd_smd_set_shape: 0..3 */
   double ms_per_step;        /* RK4 step of the whole batch, chosen form */
   double default_ms_per_step;/* same, form chosen at creation */
   double gemm_ms;            /* one trajectory-chunk contraction, chosen tile (-1: no DSSF) */
@@ -299,8 +299,9 @@ typedef struct hsmd_smd_tuning {
  * runs one sample per rank).  form: 0 = keep, 1 = LDS-resident with the
  * batch's largest slot class, 2 = LDS-resident with per-workgroup slot
  * classes (FP32 batches of two classes), 3 = global-memory form.  gemm:
- * 0 = 128 x 128 DSSF contraction tile, 2 = 128 x 256 (both FP32 with
- * 2 n_omega a multiple of 256), 1 = 64 x 64.  Errors when the shape does
+ * 0 = 128 x 128 DSSF contraction tile, 2 = 128 x 256, 3 = 128 x 128 with
+ * two LDS stages (all three FP32 with 2 n_omega a multiple of 256),
+ * 1 = 64 x 64.  Errors when the shape does
  * not apply to the batch. */
 hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm);
 /* Autotuner of the batched SMD step (no counterpart in the reference): on a

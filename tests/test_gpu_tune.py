@@ -109,13 +109,13 @@ def test_smd_tuner_and_shapes_keep_parity(tmp_path, monkeypatch):
     sd = H.SpinDynamics(lats, idx, dt, T, n_omega=128, tune=True, tune_steps=16)
     t = sd.tuning
     assert t["cached"] == 0 and t["n_candidates"] >= 3
-    assert t["form"] in (1, 2, 3) and t["gemm"] in (0, 1, 2)
+    assert t["form"] in (1, 2, 3) and t["gemm"] in (0, 1, 2, 3)
     assert t["ms_per_step"] <= t["default_ms_per_step"] and t["gemm_ms"] <= t["default_gemm_ms"]
     sd2 = H.SpinDynamics(lats, idx, dt, T, n_omega=128, tune=True, tune_steps=16)
     assert sd2.tuning["cached"] == 1
     # every shape gives the same dynamics and DSSF up to FP32 rounding
     res = []
-    for form, gemm in [(0, 0), (1, 1), (2, 2), (3, 1), (2, 0)]:
+    for form, gemm in [(0, 0), (1, 1), (2, 2), (3, 1), (2, 0), (1, 3)]:
         e = H.SpinDynamics(lats, idx, dt, T, n_omega=128)
         e.set_shape(form, gemm)
         e.set_samples(samples)

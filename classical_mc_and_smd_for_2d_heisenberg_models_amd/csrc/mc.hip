@@ -452,12 +452,20 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
   const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
   uint32_t qa[ZH], qb[ZH];
   int la = 0, lb = 0;
+  // the packed labels are padded to ZC slots with the zero-spin sentinel
+  // (coupling 0), so every slot is read without a branch; lanes past the
+  // colour class load row 0 and keep the sentinel
   auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab) {
     const int p = a.col_off[col] + tid;
     const bool v = p < a.col_off[col + 1];
+    const int pc = v ? p : 0;
 #pragma unroll
-    for (int k = 0; k < ZH; k++) q[k] = (v && 2 * k < Z) ? a.nb16[(size_t)k * N + p] : pad;
-    lab = v ? a.perm[p] : 0;
+    for (int k = 0; k < ZH; k++) {
+      const uint32_t w = a.nb16[(size_t)k * N + pc];
+      q[k] = v ? w : pad;
+    }
+    const int l = a.perm[pc];
+    lab = v ? l : 0;
   };
   if (total > 0) fetch(0, qa, la);
   __syncthreads();
@@ -471,14 +479,10 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
 #pragma unroll
       for (int k = 0; k < ZH; k++) {
         const int i0 = (int)(qa[k] & 0xffffu), i1 = (int)(qa[k] >> 16);
-        if (2 * k < Z) {
-          const T J = cj_xy<T>(a.cp, 2 * k), Jz = cj_z<T>(a.cp, 2 * k);
-          hx += J * sx[i0]; hy += J * sy[i0]; hz += Jz * sz[i0];
-        }
-        if (2 * k + 1 < Z) {
-          const T J = cj_xy<T>(a.cp, 2 * k + 1), Jz = cj_z<T>(a.cp, 2 * k + 1);
-          hx += J * sx[i1]; hy += J * sy[i1]; hz += Jz * sz[i1];
-        }
+        const T J0 = cj_xy<T>(a.cp, 2 * k), Jz0 = cj_z<T>(a.cp, 2 * k);
+        const T J1 = cj_xy<T>(a.cp, 2 * k + 1), Jz1 = cj_z<T>(a.cp, 2 * k + 1);
+        hx += J0 * sx[i0]; hy += J0 * sy[i0]; hz += Jz0 * sz[i0];
+        hx += J1 * sx[i1]; hy += J1 * sy[i1]; hz += Jz1 * sz[i1];
       }
       T x = sx[p], y = sy[p], z = sz[p];
       if (pass < nor) {
@@ -1019,8 +1023,10 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
     e->fast_ok = e->use_lds && !e->ising && !lat.general && (lat.N + 1) < 65535 && maxcol <= kThreads;
     e->fast = e->fast_ok;
     if (e->fast_ok) {
-      const int ZH = (lat.Z + 1) / 2;
-      std::vector<uint32_t> pk((size_t)std::max(ZH, 1) * N);
+      // (ZC + 1) / 2 words per site for the kernel's slot class ZC >= Z;
+      // slots past Z hold the sentinel N on both halves
+      const int ZH = (lat.Z + 1) / 2, ZHC = (zc_for(lat.Z) + 1) / 2;
+      std::vector<uint32_t> pk((size_t)ZHC * N, (uint32_t)N | ((uint32_t)N << 16));
       for (int k = 0; k < ZH; k++)
         for (int p = 0; p < N; p++) {
           const uint32_t lo = (uint32_t)lat.g_nb[(size_t)(2 * k) * N + p];

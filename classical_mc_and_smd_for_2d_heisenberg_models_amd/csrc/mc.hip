@@ -211,25 +211,39 @@ __device__ __forceinline__ void add_labels(double *v, int ls, int lh, int lv, in
   }
 }
 
-// FP64 bond-list energy (get_energyValue) and order-parameter sums.
-template <typename T>
-__device__ void heis_epilogue(const SweepArgs &a, int c, const T *sx, const T *sy, const T *sz,
-                              double *scratch) {
+// FP64 bond-list energy (get_energyValue) of one configuration; get(i, x,
+// y, z) reads site i of the workgroup's copy.
+template <typename G>
+__device__ void heis_energy(const SweepArgs &a, int c, double *scratch, G get) {
   const int N = a.cp.N;
   double e = 0.0;
   for (int b = threadIdx.x; b < a.bd.Nb; b += blockDim.x) {
     const int i = a.bi[b], j = a.bj[b], k = a.bc[b];
-    e += a.bd.jxy[k] * ((double)sx[i] * (double)sx[j] + (double)sy[i] * (double)sy[j]) +
-         a.bd.jz[k] * ((double)sz[i] * (double)sz[j]);
+    double xi, yi, zi, xj, yj, zj;
+    get(i, xi, yi, zi);
+    get(j, xj, yj, zj);
+    e += a.bd.jxy[k] * (xi * xj + yi * yj) + a.bd.jz[k] * (zi * zj);
   }
   if (a.cp.h_on) {
     double mz = 0.0;
-    for (int p = threadIdx.x; p < N; p += blockDim.x) mz += (double)sz[p];
+    for (int p = threadIdx.x; p < N; p += blockDim.x) {
+      double x, y, z;
+      get(p, x, y, z);
+      mz += z;
+    }
     e -= a.cp.h * mz;
   }
   double v1[1] = {e};
   block_sum<1>(v1, scratch);
   if (threadIdx.x == 0) a.energy[c] = v1[0];
+}
+
+template <typename T>
+__device__ void heis_epilogue(const SweepArgs &a, int c, const T *sx, const T *sy, const T *sz,
+                              double *scratch) {
+  heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {
+    x = (double)sx[i]; y = (double)sy[i]; z = (double)sz[i];
+  });
 }
 
 // Order-parameter sums of one configuration per workgroup, read back from
@@ -429,17 +443,35 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a)
 // neighbour labels packed as 16-bit pairs.  The labels of the next class are
 // fetched before the current class is updated, so their L2 latency overlaps
 // the gathers, the update and the class barrier.
-template <typename T, int ZC>
+// AOS (FP32 only): the LDS copy holds each site as one float4 (x, y, z, 0),
+// so a neighbour is one 16-byte LDS read at one address instead of three
+// reads at three addresses; otherwise three component arrays.
+template <typename T, int ZC, bool AOS>
 __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
+  static_assert(!AOS || sizeof(T) == 4, "the float4 layout is FP32 only");
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBS * 16];
   constexpr int ZH = (ZC + 1) / 2;
   const int c = blockIdx.x;
-  const int N = a.cp.N, Npad = a.cp.Npad, Z = a.cp.Z;
+  const int N = a.cp.N, Npad = a.cp.Npad;
   T *g = (T *)a.spins + (size_t)c * 3 * Npad;
   T *s = (T *)smem;
-  for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) s[i] = g[i];
+  float4 *s4 = (float4 *)smem;
+  if constexpr (AOS) {
+    for (int i = threadIdx.x; i < Npad; i += blockDim.x)
+      s4[i] = make_float4(g[i], g[Npad + i], g[2 * Npad + i], 0.f);
+  } else {
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) s[i] = g[i];
+  }
   T *sx = s, *sy = s + Npad, *sz = s + 2 * Npad;
+  auto get = [&](int i, T &x, T &y, T &z) {
+    if constexpr (AOS) {
+      const float4 v = s4[i];
+      x = v.x; y = v.y; z = v.z;
+    } else {
+      x = sx[i]; y = sy[i]; z = sz[i];
+    }
+  };
   const int slot = a.slot_of_config[c];
   const double beta = a.beta_slot[slot];
   const bool boost = a.boost_slot[slot] != 0;
@@ -481,10 +513,14 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
         const int i0 = (int)(qa[k] & 0xffffu), i1 = (int)(qa[k] >> 16);
         const T J0 = cj_xy<T>(a.cp, 2 * k), Jz0 = cj_z<T>(a.cp, 2 * k);
         const T J1 = cj_xy<T>(a.cp, 2 * k + 1), Jz1 = cj_z<T>(a.cp, 2 * k + 1);
-        hx += J0 * sx[i0]; hy += J0 * sy[i0]; hz += Jz0 * sz[i0];
-        hx += J1 * sx[i1]; hy += J1 * sy[i1]; hz += Jz1 * sz[i1];
+        T x0, y0, z0, x1, y1, z1;
+        get(i0, x0, y0, z0);
+        get(i1, x1, y1, z1);
+        hx += J0 * x0; hy += J0 * y0; hz += Jz0 * z0;
+        hx += J1 * x1; hy += J1 * y1; hz += Jz1 * z1;
       }
-      T x = sx[p], y = sy[p], z = sz[p];
+      T x, y, z;
+      get(p, x, y, z);
       if (pass < nor) {
         overrelax<T>(hx, hy, hz, x, y, z);
       } else {
@@ -492,15 +528,27 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
         const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
         heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
       }
-      sx[p] = x; sy[p] = y; sz[p] = z;
+      if constexpr (AOS) s4[p] = make_float4(x, y, z, 0.f);
+      else { sx[p] = x; sy[p] = y; sz[p] = z; }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ZH; k++) qa[k] = qb[k];
     la = lb;
   }
-  heis_epilogue<T>(a, c, sx, sy, sz, scratch);
-  for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) g[i] = s[i];
+  heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {
+    T u, v, w;
+    get(i, u, v, w);
+    x = (double)u; y = (double)v; z = (double)w;
+  });
+  if constexpr (AOS) {
+    for (int i = threadIdx.x; i < Npad; i += blockDim.x) {
+      const float4 v = s4[i];
+      g[i] = v.x; g[Npad + i] = v.y; g[2 * Npad + i] = v.z;
+    }
+  } else {
+    for (int i = threadIdx.x; i < 3 * Npad; i += blockDim.x) g[i] = s[i];
+  }
 }
 
 // --------------------------------------------------------------- Ising
@@ -798,6 +846,8 @@ struct MCEngine {
   int threads = kThreads;
   bool fast_ok = false;
   int maxcol = 0;
+  bool aos = false;       // FP32 fast path with float4 sites in LDS
+  size_t aos_bytes = 0;
   DevBuf<uint8_t> bc;
   DevBuf<int8_t> lab;
   DevBuf<float> Jf;
@@ -1022,6 +1072,8 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
     e->maxcol = maxcol;
     e->fast_ok = e->use_lds && !e->ising && !lat.general && (lat.N + 1) < 65535 && maxcol <= kThreads;
     e->fast = e->fast_ok;
+    e->aos_bytes = (size_t)(N + 1) * sizeof(float4);
+    e->aos = e->fast_ok && f32 && e->aos_bytes <= lds_limit();
     if (e->fast_ok) {
       // (ZC + 1) / 2 words per site for the kernel's slot class ZC >= Z;
       // slots past Z hold the sentinel N on both halves
@@ -1127,8 +1179,10 @@ static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
     if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
   } else if (e.fast && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
-    if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
-    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    if (f32 && e.aos)
+      hipLaunchKernelGGL((heis_sweep_fast<float, ZC, true>), dim3(e.R), dim3(e.threads), e.aos_bytes, st, a);
+    else if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC, false>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC, false>), dim3(e.R), dim3(e.threads), sh, st, a);
   } else if (f32) {
     if (e.use_lds) hipLaunchKernelGGL((heis_sweep_kernel<float, true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((heis_sweep_kernel<float, false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
@@ -1147,8 +1201,12 @@ static void set_lds_attr(MCEngine &e) {
   else fn = (const void *)heis_sweep_kernel<double, true, ZC>;
   HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
   if (e.fast_ok) {
-    fn = f32 ? (const void *)heis_sweep_fast<float, ZC> : (const void *)heis_sweep_fast<double, ZC>;
+    fn = f32 ? (const void *)heis_sweep_fast<float, ZC, false> : (const void *)heis_sweep_fast<double, ZC, false>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+  }
+  if (e.aos) {
+    fn = (const void *)heis_sweep_fast<float, ZC, true>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.aos_bytes));
   }
 }
 

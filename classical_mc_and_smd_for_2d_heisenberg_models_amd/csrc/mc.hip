@@ -484,27 +484,31 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
   const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
   uint32_t qa[ZH], qb[ZH];
   int la = 0, lb = 0;
+  bool vb = false;
   // the packed labels are padded to ZC slots with the zero-spin sentinel
-  // (coupling 0), so every slot is read without a branch; lanes past the
-  // colour class load row 0 and keep the sentinel
-  auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab) {
+  // (coupling 0), so every slot is read without a branch.  The next class's
+  // labels are loaded unconditionally (lanes past the class read row 0) and
+  // replaced by the sentinel only after the barrier, so the loads stay in
+  // flight across the current class's update
+  auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab, bool &v) {
     const int p = a.col_off[col] + tid;
-    const bool v = p < a.col_off[col + 1];
+    v = p < a.col_off[col + 1];
     const int pc = v ? p : 0;
 #pragma unroll
-    for (int k = 0; k < ZH; k++) {
-      const uint32_t w = a.nb16[(size_t)k * N + pc];
-      q[k] = v ? w : pad;
-    }
-    const int l = a.perm[pc];
-    lab = v ? l : 0;
+    for (int k = 0; k < ZH; k++) q[k] = a.nb16[(size_t)k * N + pc];
+    lab = a.perm[pc];
   };
-  if (total > 0) fetch(0, qa, la);
+  if (total > 0) {
+    bool va;
+    fetch(0, qa, la, va);
+#pragma unroll
+    for (int k = 0; k < ZH; k++) qa[k] = va ? qa[k] : pad;
+  }
   __syncthreads();
   for (int it = 0; it < total; it++) {
     const int pass = it / nc;
     const int col = it - pass * nc;
-    if (it + 1 < total) fetch((it + 1) % nc, qb, lb);
+    if (it + 1 < total) fetch((it + 1) % nc, qb, lb, vb);
     const int p = a.col_off[col] + tid;
     if (p < a.col_off[col + 1]) {
       T hx = T(0), hy = T(0), hz = hz0;
@@ -533,7 +537,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < ZH; k++) qa[k] = qb[k];
+    for (int k = 0; k < ZH; k++) qa[k] = vb ? qb[k] : pad;
     la = lb;
   }
   heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {

@@ -482,6 +482,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
   const int tid = threadIdx.x;
   const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
   const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
+  const uint32_t sweep16 = sweep_of(a) * 16u;   // read once, before the label loads
   uint32_t qa[ZH], qb[ZH];
   int la = 0, lb = 0;
   bool vb = false;
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
       if (pass < nor) {
         overrelax<T>(hx, hy, hz, x, y, z);
       } else {
-        const uint32_t step = sweep_of(a) * 16u + (uint32_t)pass;
+        const uint32_t step = sweep16 + (uint32_t)pass;
         const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
         heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
       }
@@ -537,8 +538,14 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < ZH; k++) qa[k] = vb ? qb[k] : pad;
+    for (int k = 0; k < ZH; k++) {
+      qa[k] = vb ? qb[k] : pad;
+      // settled here: no later wait on the memory counter (which would also
+      // cover the next class's loads) is needed for these registers
+      asm volatile("" : "+v"(qa[k]));
+    }
     la = lb;
+    asm volatile("" : "+v"(la));
   }
   heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {
     T u, v, w;

@@ -442,7 +442,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // launch shape
   e->spt = 1;
   while (e->spt * kSmdThreads < maxN) e->spt *= 2;
-  e->lds_npad = maxN + 1;
+  e->lds_npad = e->spt * kSmdThreads + 1;   // the kernels' compile-time component stride
   e->lds_bytes = (size_t)3 * 3 * e->lds_npad * es;   // S, B1, B2
   e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
                e->spt * (int)es <= 16;

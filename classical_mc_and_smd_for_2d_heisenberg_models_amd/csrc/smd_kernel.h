@@ -74,7 +74,7 @@ struct StepArgs {
   int n_points;         // time points produced by this launch
   int chunk_base;       // time index stored at trajectory row 0
   int T;                // total time points
-  int lds_npad;         // LDS stride (max Npad) when LDS is used
+  int lds_npad;         // LDS component stride, SPT x 1024 + 1 (LDS form)
   int sample_base;      // first sample of this launch
 };
 
@@ -190,7 +190,9 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
   T *S, *B1, *B2;
   int stride;
   if (LDS) {
-    stride = a.lds_npad;
+    // a compile-time component stride: the y and z reads of a neighbour then
+    // take the x address plus an immediate LDS offset
+    stride = SPT * kSmdThreads + 1;
     S = (T *)smem;
     B1 = S + 3 * stride;
     B2 = B1 + 3 * stride;

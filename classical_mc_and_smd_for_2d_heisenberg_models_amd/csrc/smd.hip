@@ -377,7 +377,9 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
     // 16-bit neighbour labels, two slots per word, kept in registers by the kernel
     if (L.N + 1 >= 65535) throw Error(HSMD_ERR_ARG, "SMD lattices are limited to 65534 sites");
     {
-      const int ZH = std::max(1, (L.Z + 1) / 2);
+      // padded to kZMAX / 2 words per site with the sentinel, so a kernel of
+      // any slot class reads every word of its class without a bounds test
+      const int ZH = kZMAX / 2;
       std::vector<uint32_t> pk((size_t)ZH * L.N, (uint32_t)L.N | ((uint32_t)L.N << 16));
       for (int k = 0; 2 * k < L.Z; k++)
         for (int p = 0; p < L.N; p++) {

@@ -13,12 +13,19 @@ struct LatDev {
   Coup cp;
   Bonds bd;
   const int32_t *nb;
-  const uint32_t *nb16;   // packed 16-bit neighbour pairs, (Z+1)/2 x N
+  const uint32_t *nb16;   // packed 16-bit neighbour pairs, kZMAX/2 x N (sentinel-padded)
   const void *Jt;
   const int32_t *bi, *bj;
   const uint8_t *bc;
   int field_energy;       // 1: bond energy == 1/2 sum S.h - 1/2 h sum Sz (symmetric tables)
 };
+
+// Load through a global-address-space pointer (a plain global load rather
+// than a flat one that must also consider LDS).
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
 
 struct SampleDev {
   int li;
@@ -218,7 +225,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
     const int p = threadIdx.x + u * kSmdThreads;
     if constexpr (QREG) {
 #pragma unroll
-      for (int k = 0; k < ZH; k++) q[u][k] = (p < N && 2 * k < Z) ? nb16[(size_t)k * N + p] : pad;
+      for (int k = 0; k < ZH; k++) q[u][k] = p < N ? gload(nb16 + (size_t)k * N + p) : pad;
     }
     if constexpr (GREG) {
 #pragma unroll
@@ -242,7 +249,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
       int pp = p;
       asm volatile("" : "+v"(pp));
 #pragma unroll
-      for (int k = 0; k < ZH; k++) w[k] = (2 * k < Z) ? nb16[(size_t)k * N + pp] : pad;
+      for (int k = 0; k < ZH; k++) w[k] = gload(nb16 + (size_t)k * N + pp);
     }
     field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
   };
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
     int pp = p;
     asm volatile("" : "+v"(pp));
 #pragma unroll
-    for (int k = 0; k < ZH; k++) w[k] = (2 * k < Z) ? nb16[(size_t)k * N + pp] : pad;
+    for (int k = 0; k < ZH; k++) w[k] = gload(nb16 + (size_t)k * N + pp);
     field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
   };
   double qss_prev = 0.0;

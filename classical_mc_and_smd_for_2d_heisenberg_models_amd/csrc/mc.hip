@@ -491,13 +491,19 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
   // labels are loaded unconditionally (lanes past the class read row 0) and
   // replaced by the sentinel only after the barrier, so the loads stay in
   // flight across the current class's update
+  // buffer loads: the lane's site is the per-lane offset, the word row the
+  // scalar offset, so the loads need no per-lane address arithmetic
+  const __amdgpu_buffer_rsrc_t nrsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.nb16, (short)0, 0x7fffffff, 0x00020000);
+  const __amdgpu_buffer_rsrc_t prsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.perm, (short)0, 0x7fffffff, 0x00020000);
   auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab, bool &v) {
     const int p = a.col_off[col] + tid;
     v = p < a.col_off[col + 1];
-    const int pc = v ? p : 0;
+    const int pc4 = 4 * (v ? p : 0);
 #pragma unroll
-    for (int k = 0; k < ZH; k++) q[k] = a.nb16[(size_t)k * N + pc];
-    lab = a.perm[pc];
+    for (int k = 0; k < ZH; k++) q[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, pc4, 4 * k * N, 0);
+    lab = __builtin_amdgcn_raw_buffer_load_b32(prsrc, pc4, 0, 0);
   };
   if (total > 0) {
     bool va;

@@ -218,10 +218,18 @@ def cpu_baseline(lat, c, temps, seconds_target=15.0):
                    threads=R)
     dt = time.perf_counter() - t0
     ups = R * lat.N * 6 * n / dt
+    # single core: one replica, one thread, a few seconds
+    n1 = max(1, min(50, int(0.25 * seconds_target / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    m.mc_reference(0, temps[sel[:1]], s[:1].copy(), n1, n_or=5, n_hb=1, n_or_b=5, n_hb_b=1,
+                   pt_start=0, threads=1)
+    d1 = time.perf_counter() - t0
     return {"value": ups, "unit": "spin-updates/s", "cores": R, "kind": "port",
+            "single_core": lat.N * 6 * n1 / d1,
             "sample": "reference-order FP64 sweeps (random permutation, 5 OR + 1 HB, bond-list "
                       "energy, replica exchange) of %d replicas of the triangular L=96 lattice "
-                      "for %d sweeps, one OpenMP thread per replica" % (R, n)}
+                      "for %d sweeps, one OpenMP thread per replica; single_core: one replica, "
+                      "%d sweeps, one thread" % (R, n, n1)}
 
 
 def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt", key="traffic_bytes_per_launch"):

@@ -79,6 +79,11 @@ struct SweepArgs {
   const double *qc_ref;
   const double2 *qc_phase;
   double *qc_obs;
+  // fast kernels: energy from the neighbour table over the slots of emask,
+  // weighted by escale (table_energy != 0), instead of the bond list
+  int table_energy;
+  uint32_t emask;
+  double escale;
 };
 
 __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
@@ -553,11 +558,49 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
     la = lb;
     asm volatile("" : "+v"(la));
   }
-  heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {
-    T u, v, w;
-    get(i, u, v, w);
-    x = (double)u; y = (double)v; z = (double)w;
-  });
+  if (a.table_energy) {
+    // E = escale sum_i S_i . sum_{s in emask} (J_s S_nb(i,s)) - h sum_i z_i,
+    // every product and sum in FP64; the labels are read again (coalesced,
+    // from L2) and the spins from LDS
+    double e = 0.0, mz = 0.0;
+    for (int p = tid; p < N; p += blockDim.x) {
+      uint32_t q[ZH];
+#pragma unroll
+      for (int k = 0; k < ZH; k++) q[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, 4 * p, 4 * k * N, 0);
+      double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+      for (int k = 0; k < ZH; k++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int sl = 2 * k + h;
+          if (a.emask & (1u << sl)) {
+            const int j = h ? (int)(q[k] >> 16) : (int)(q[k] & 0xffffu);
+            T u, v, w;
+            get(j, u, v, w);
+            ax = fma(a.cp.jxy[sl], (double)u, ax);
+            ay = fma(a.cp.jxy[sl], (double)v, ay);
+            az = fma(a.cp.jz[sl], (double)w, az);
+          }
+        }
+      T x, y, z;
+      get(p, x, y, z);
+      e = fma((double)x, ax, e);
+      e = fma((double)y, ay, e);
+      e = fma((double)z, az, e);
+      mz += (double)z;
+    }
+    e *= a.escale;
+    if (a.cp.h_on) e -= a.cp.h * mz;
+    double v1[1] = {e};
+    block_sum<1>(v1, scratch);
+    if (threadIdx.x == 0) a.energy[c] = v1[0];
+  } else {
+    heis_energy(a, c, scratch, [&](int i, double &x, double &y, double &z) {
+      T u, v, w;
+      get(i, u, v, w);
+      x = (double)u; y = (double)v; z = (double)w;
+    });
+  }
   if constexpr (AOS) {
     for (int i = threadIdx.x; i < Npad; i += blockDim.x) {
       const float4 v = s4[i];
@@ -710,7 +753,7 @@ __global__ void measure_exchange_kernel(const PTArgs a) {
   const uint32_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
   const int parity = (a.sweep_dev && a.parity_alt) ? (int)(sweep % 2u) : a.parity;
   const double iN = 1.0 / (double)a.N;
-  if (t < a.R) {
+  if (t < a.R && (a.measure || a.etrace)) {
     const int k = t;
     const int c = a.config_of_slot[k];
     const double E = a.energy[c];
@@ -941,6 +984,69 @@ void fill_bonds(const Lattice &L, Bonds &bd) {
   for (int k = 0; k < bd.n_class; k++) { bd.jxy[k] = L.bc_Jxy[k]; bd.jz[k] = L.bc_Jz[k]; }
 }
 
+// Energy form of the fast sweep kernels.  The bond-list energy
+// sum_b J_b (x_i x_j + y_i y_j) + Jz_b z_i z_j is a quadratic form in the
+// spins; the table form w sum_i sum_{s in F} S_i . (J_s, J_s, Jz_s) S_nb(i,s)
+// is another.  F = the first half of every shell's slots with w = 1 (each
+// bond met once: centrally symmetric shells of a Bravais lattice in
+// descending-angle order), else all slots with w = 1/2.  A form is taken
+// only if it equals the bond list's, checked on a pseudo-random
+// configuration (two different forms differ there by far more than the
+// rounding tolerance); otherwise the kernels keep the bond list.  Sparse
+// tables (irregular graphs, where most of the Z slots of a site hold the
+// sentinel) also keep the bond list, which visits only real bonds.
+static void pick_energy_form(const Lattice &L, const Coup &cp, SweepArgs &a) {
+  a.table_energy = 0;
+  a.emask = 0u;
+  a.escale = 1.0;
+  if (L.general || L.Z > 32) return;
+  const int N = L.N;
+  size_t filled = 0;
+  for (size_t k = 0; k < (size_t)L.Z * N; k++) filled += L.g_nb[k] < N ? 1 : 0;
+  if (2 * filled < (size_t)L.Z * N) return;
+  std::vector<double> S((size_t)3 * (N + 1), 0.0);
+  uint32_t st = 0x2545f491u;
+  auto rnd = [&]() { st ^= st << 13; st ^= st >> 17; st ^= st << 5; return (st >> 8) * (1.0 / 16777216.0) - 0.5; };
+  for (int i = 0; i < N; i++) {
+    double v[3] = {rnd(), rnd(), rnd()};
+    const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + 1e-12;
+    for (int d = 0; d < 3; d++) S[3 * i + d] = v[d] / n;
+  }
+  double eb = 0.0;
+  for (size_t b = 0; b < L.g_bi.size(); b++) {
+    const int i = L.g_bi[b], j = L.g_bj[b], k = L.g_bc[b];
+    eb += L.bc_Jxy[k] * (S[3 * i] * S[3 * j] + S[3 * i + 1] * S[3 * j + 1]) +
+          L.bc_Jz[k] * (S[3 * i + 2] * S[3 * j + 2]);
+  }
+  auto table = [&](uint32_t mask, double w) {
+    double e = 0.0;
+    for (int p = 0; p < N; p++)
+      for (int s = 0; s < L.Z; s++) {
+        if (!(mask & (1u << s))) continue;
+        const int q = L.g_nb[(size_t)s * N + p];
+        e += cp.jxy[s] * (S[3 * p] * S[3 * q] + S[3 * p + 1] * S[3 * q + 1]) +
+             cp.jz[s] * (S[3 * p + 2] * S[3 * q + 2]);
+      }
+    return w * e;
+  };
+  const double tol = 1e-9 * (1.0 + fabs(eb));
+  int count[4] = {0, 0, 0, 0};
+  for (int s = 0; s < L.Z; s++) count[L.slot_shell[s]]++;
+  uint32_t half = 0u, all = 0u;
+  bool even = true;
+  for (int s = 0; s < L.Z; s++) {
+    all |= 1u << s;
+    const int n = count[L.slot_shell[s]];
+    if (n % 2) even = false;
+    if (L.slot_k[s] < n / 2) half |= 1u << s;
+  }
+  if (even && fabs(table(half, 1.0) - eb) <= tol) {
+    a.table_energy = 1; a.emask = half; a.escale = 1.0;
+  } else if (fabs(table(all, 0.5) - eb) <= tol) {
+    a.table_energy = 1; a.emask = all; a.escale = 0.5;
+  }
+}
+
 static size_t lds_limit() { return 160 * 1024 - NOBS * 16 * 8 - (kBCMAX + 1) * 16 * 8 - 1024; }
 
 template <int ZC>
@@ -1104,6 +1210,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
         }
       e->nb16.upload(pk.data(), pk.size());
       a.nb16 = e->nb16.p;
+      pick_energy_form(lat, a.cp, a);
     }
   }
   if (e->use_lds) HSMD_ZC_DISPATCH(lat.Z, set_lds_attr, *e);

@@ -49,6 +49,36 @@ FIELD_CASES = [
 ]
 
 
+SWEEP_ENERGY_CASES = [
+    ("square", 24, dict(J1=1.0, J2=0.6), True),
+    ("triang", 12, dict(J1=1.0, J2=0.12, JX=0.05, h=0.3, lambda1=0.9, lambda2=0.9), True),
+    ("kagome", 6, dict(J1=1.0, J2=-0.2, JX=0.1, lambda1=1.2, lambda2=0.8, h=-0.4), True),
+    ("hexagn", 6, dict(J1=0.7, J2=0.3, JX=0.2), True),
+    ("square", 8, dict(J1=1.0, J2=0.4, h=0.2), False),
+    ("triang", 6, dict(J1=1.0, J2=0.3), True),
+]
+
+
+@pytest.mark.parametrize("prec", [H.BATCHED_F32, H.BATCHED_F64])
+@pytest.mark.parametrize("geom,L,kw,pbc", SWEEP_ENERGY_CASES)
+def test_sweep_energies_equal_bond_list_energies(geom, L, kw, pbc, prec):
+    """The energy each sweep kernel reports (table form or bond list, as the
+    engine picks) equals get_energyValue's bond-list sum over the spins the
+    sweep left, in FP64 (products of the stored spins), to 1e-12."""
+    _say("sweep energy", geom, L, pbc, prec)
+    c = H.Couplings(**kw)
+    lat = H.Lattice.generate(geom, L, c, pbc=pbc)
+    temps = np.array([0.2, 0.7, 1.4, 2.5, 4.0])
+    rng = np.random.default_rng(31)
+    mc = H.MonteCarlo(lat, temps, precision=prec, seed=4)
+    mc.set_spins(_spins(rng, len(temps), lat.N))
+    for n, parity in [(1, -1), (3, 2), (2, 0)]:
+        mc.run(n, pt_parity=parity)
+        E = mc.energies()
+        ref = lat.energies(mc.get_spins(), prec)
+        np.testing.assert_allclose(E, ref, rtol=1e-12, atol=1e-11)
+
+
 @pytest.mark.parametrize("geom,L,kw,imp", FIELD_CASES)
 def test_local_fields_and_energies_fp64_match_oracle(geom, L, kw, imp):
     _say("fields", geom, L, kw)

@@ -374,3 +374,85 @@ def test_observables_match_reference_order_chain_within_error_bars():
             within2 += abs(ga - ca) < 2 * sig
             total += 1
     assert within2 >= 0.8 * total
+
+
+@pytest.mark.parametrize("R", [1, 7, 1024])
+def test_ising_replay_at_replica_count_edges(R):
+    """One replica (no exchange pairs), an odd count (the last slot has no
+    partner on one parity) and the largest count (1024, one exchange thread
+    per pair, a full grid of sweep workgroups): bit-exact against the
+    oracle's colour replay with alternating pair parity."""
+    _say("ising replay R", R)
+    c = H.Couplings(J1=1.0, J2=0.3)
+    lat = H.Lattice.generate("square", 6 if R == 1024 else 10, c)
+    m = _model(lat, c, "square")
+    temps = np.geomspace(0.5, 5.0, R)
+    rng = np.random.default_rng(R)
+    s = np.zeros((R, lat.N, 3))
+    s[:, :, 0] = rng.choice([-1.0, 1.0], size=(R, lat.N))
+    mc = H.MonteCarlo(lat, temps, model=H.ISING, precision=H.BATCHED_F32, seed=5)
+    mc.set_spins(s)
+    n = 12
+    tr = mc.run(n, pt_parity=2, trace=True)
+    ref = s.copy()
+    rtr, (cos, cnt, exc, boost) = m.mc_replay(1, temps, lat.colours(), ref, n, seed=5, pt_parity=2)
+    np.testing.assert_array_equal(tr, rtr)
+    np.testing.assert_array_equal(mc.slot_map(), cos)
+    att, acc = mc.pt_counters()
+    np.testing.assert_array_equal(att, cnt)
+    np.testing.assert_array_equal(acc, exc)
+    np.testing.assert_array_equal(mc.get_spins(), ref[cos])
+    if R == 1:
+        assert att.sum() == 0
+    else:
+        assert acc.sum() > 0
+    # replayed graph form of the same chain
+    mc.run(6, pt_parity=2)
+    m.mc_replay(1, temps, lat.colours(), ref, 6, seed=5, sweep0=n, pt_parity=2,
+                state=(cos, cnt, exc, boost))
+    np.testing.assert_array_equal(mc.get_spins(), ref[cos])
+
+
+def test_replica_count_limits():
+    _say("replica limits")
+    lat = H.Lattice.generate("square", 4, H.Couplings(J1=1.0))
+    with pytest.raises(H.HSMDError):
+        H.MonteCarlo(lat, np.linspace(1.0, 2.0, 1025))
+    with pytest.raises(H.HSMDError):
+        H.MonteCarlo(lat, np.zeros(0))
+
+
+def test_heisenberg_fp64_beyond_lds_matches_colour_replay():
+    """A replica too large for LDS in FP64 (square L=96: 3 x 9217 x 8 B =
+    221 KB) runs the global-memory sweep; it matches the oracle's FP64
+    colour replay as the LDS-resident form does."""
+    _say("heis fp64 beyond LDS")
+    c = H.Couplings(J1=1.0, J2=0.6, h=0.2, lambda1=0.9, lambda2=1.1)
+    lat = H.Lattice.generate("square", 96, c)
+    m = _model(lat, c)
+    temps = np.array([0.3, 1.5])
+    rng = np.random.default_rng(8)
+    s = _spins(rng, 2, lat.N)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F64, n_or=2, n_hb=2, seed=11)
+    mc.set_spins(s)
+    tr = mc.run(2, pt_parity=-1, trace=True)
+    ref = s.copy()
+    rtr, _ = m.mc_replay(0, temps, lat.colours(), ref, 2, n_or=2, n_hb=2, seed=11)
+    np.testing.assert_allclose(mc.get_spins(), ref, atol=1e-9)
+    np.testing.assert_allclose(tr, rtr, rtol=1e-10)
+
+
+def test_fp32_beyond_lds_energies_and_unit_spins():
+    """FP32 replicas too large for LDS (square L=128, 197 KB) sweep in global
+    memory: unit spins, and energies equal to the bond-list sum of the spins."""
+    _say("fp32 beyond LDS")
+    c = H.Couplings(J1=1.0, J2=0.5, JX=0.1, h=0.1)
+    lat = H.Lattice.generate("square", 128, c)
+    temps = np.array([0.4, 1.0, 3.0])
+    rng = np.random.default_rng(9)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, seed=12)
+    mc.set_spins(_spins(rng, 3, lat.N))
+    mc.run(4, pt_parity=2)
+    sp = mc.get_spins()
+    np.testing.assert_allclose(np.linalg.norm(sp, axis=-1), 1.0, atol=1e-5)
+    np.testing.assert_allclose(mc.energies(), lat.energies(sp, H.BATCHED_F32), rtol=1e-12, atol=1e-10)

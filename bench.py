@@ -125,6 +125,40 @@ def smd_workload(rng):
     return lats, idx, samples
 
 
+def bench_measure(timer, mc, lat, R, sweep_ms, k):
+    """Measurement-stage sweeps on the headline engine: order parameters,
+    spin-stiffness bond sums and per-slot moments every sweep (obs_kernel and
+    the moments in the exchange kernel), and with the static S(q) of every
+    replica every 10th measured sweep (grid DFT, FP64).  Costs are differences
+    against the unmeasured sweep; algorithmic bytes of obs_kernel = 12 B per
+    site + 2 x 12 B per bond per replica, FLOP of one S(q) = 8 x 2 x 3 x G^3
+    per replica (two separable complex DFT passes over a G x G grid)."""
+    N, nb = lat.N, lat.n_bonds
+    mc.run(5, pt_parity=2, measure=True)
+    timer.start()
+    mc.run(k, pt_parity=2, measure=True)
+    m_ms = timer.stop() / k
+    mc.set_sq(10)
+    mc.run(10, pt_parity=2, measure=True)
+    timer.start()
+    mc.run(10 * (k // 10), pt_parity=2, measure=True)
+    q_ms = timer.stop() / (10 * (k // 10))
+    mc.set_sq(0)
+    obs_ms = max(m_ms - sweep_ms, 1e-6)
+    sq_ms = max(10 * (q_ms - m_ms), 1e-6)
+    G = lat.Gsz
+    obs_bytes = (12.0 * N + 24.0 * nb) * R
+    sq_flop = 8.0 * 2 * 3 * G ** 3 * R
+    print("[bench] MC measured sweep %.3f ms (observables +%.3f ms); with S(q) every 10: %.3f ms "
+          "(one S(q) of %d replicas %.3f ms)" % (m_ms, obs_ms, q_ms, R, sq_ms),
+          file=sys.stderr, flush=True)
+    return {"ms_per_measured_sweep": m_ms, "observables_ms": obs_ms,
+            "observables_gbs": obs_bytes / (obs_ms * 1e-3) / 1e9,
+            "ms_per_sweep_with_sq_every_10": q_ms, "sq_ms_per_call": sq_ms,
+            "sq_grid": G, "sq_tflops_fp64": sq_flop / (sq_ms * 1e-3) / 1e12,
+            "sweeps": k}
+
+
 def graph_workload(rng, n_target=12000):
     """Poisson-disk graph of about n_target sites (Bridson sampling fills a
     periodic box at ~0.627 points per r_min^2) with 5 % vacancies."""
@@ -348,6 +382,7 @@ def main():
                      "kernel": mc_kernel,
                      "bytes_per_update": bytes_per_update, "kernel_ms": kms},
     }
+    out["measure"] = bench_measure(timer, mc, lat, R, ms / args.steps, max(10, args.steps // 4))
     del mc
     if not args.no_plain:
         mp = H.MonteCarlo(lat, temps, precision=H.PLAIN_F64, n_or=5, n_hb=1, n_or_boost=5,

@@ -269,13 +269,26 @@ __device__ __forceinline__ void load_spin<int8_t>(const int8_t *g, int Npad, int
 // (CF/subrouts.cpp:6519-6620): dprod = J (x_i x_j + y_i y_j),
 // vprod = J (x_i y_j - y_i x_j), weighted by C^2 and C for the three lattice
 // directions (weights precomputed per bond).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
+// 512 threads: the 40 FP64 accumulators per thread stay in registers (at
+// 1024 threads per workgroup the register limit of 128 forces them to
+// scratch memory).
+// With LDS the configuration is staged into LDS first, so the bond and
+// plaquette gathers read LDS instead of L2.
+constexpr int kObsThreads = 512;
+template <typename T, bool LDS>
+__global__ __launch_bounds__(kObsThreads) void obs_kernel(const SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBSX * 16];
   const int c = blockIdx.x;
   const int N = a.cp.N, Npad = a.cp.Npad;
   const int comps = sizeof(T) == 1 ? 1 : 3;
   const T *g = (const T *)a.spins + (size_t)c * comps * Npad;
+  if constexpr (LDS) {
+    T *sm = (T *)smem;
+    for (int i = threadIdx.x; i < comps * Npad; i += blockDim.x) sm[i] = g[i];
+    __syncthreads();
+    g = sm;
+  }
   double v[NOBSX];
 #pragma unroll
   for (int k = 0; k < NOBSX; k++) v[k] = 0.0;
@@ -298,6 +311,7 @@ __global__ __launch_bounds__(kThreads) void obs_kernel(const SweepArgs a) {
     }
   }
   const int Nb = a.bd.Nb;
+#pragma unroll 4
   for (int b = threadIdx.x; b < Nb; b += blockDim.x) {
     double xi, yi, zi, xj, yj, zj;
     load_spin<T>(g, Npad, a.bi[b], xi, yi, zi);
@@ -1324,6 +1338,10 @@ static void set_lds_attr(MCEngine &e) {
   else if (f32) fn = (const void *)heis_sweep_kernel<float, true, ZC>;
   else fn = (const void *)heis_sweep_kernel<double, true, ZC>;
   HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+  if (e.ising) fn = (const void *)obs_kernel<int8_t, true>;
+  else if (f32) fn = (const void *)obs_kernel<float, true>;
+  else fn = (const void *)obs_kernel<double, true>;
+  HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
   if (e.fast_ok) {
     fn = f32 ? (const void *)heis_sweep_fast<float, ZC, false> : (const void *)heis_sweep_fast<double, ZC, false>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
@@ -1339,10 +1357,17 @@ static void launch_sweep(MCEngine &e, hipStream_t st) {
 }
 
 static void launch_obs(MCEngine &e, hipStream_t st) {
-  if (e.ising) hipLaunchKernelGGL(obs_kernel<int8_t>, dim3(e.R), dim3(kThreads), 0, st, e.args);
-  else if (e.prec == HSMD_BATCHED_F32)
-    hipLaunchKernelGGL(obs_kernel<float>, dim3(e.R), dim3(kThreads), 0, st, e.args);
-  else hipLaunchKernelGGL(obs_kernel<double>, dim3(e.R), dim3(kThreads), 0, st, e.args);
+  const size_t sh = e.use_lds ? e.lds_bytes : 0;
+  if (e.ising) {
+    if (e.use_lds) hipLaunchKernelGGL((obs_kernel<int8_t, true>), dim3(e.R), dim3(kObsThreads), sh, st, e.args);
+    else hipLaunchKernelGGL((obs_kernel<int8_t, false>), dim3(e.R), dim3(kObsThreads), 0, st, e.args);
+  } else if (e.prec == HSMD_BATCHED_F32) {
+    if (e.use_lds) hipLaunchKernelGGL((obs_kernel<float, true>), dim3(e.R), dim3(kObsThreads), sh, st, e.args);
+    else hipLaunchKernelGGL((obs_kernel<float, false>), dim3(e.R), dim3(kObsThreads), 0, st, e.args);
+  } else {
+    if (e.use_lds) hipLaunchKernelGGL((obs_kernel<double, true>), dim3(e.R), dim3(kObsThreads), sh, st, e.args);
+    else hipLaunchKernelGGL((obs_kernel<double, false>), dim3(e.R), dim3(kObsThreads), 0, st, e.args);
+  }
   if (e.args.qc) {
     if (e.ising) hipLaunchKernelGGL(qc_obs_kernel<int8_t>, dim3(e.R), dim3(256), 0, st, e.args);
     else if (e.prec == HSMD_BATCHED_F32)

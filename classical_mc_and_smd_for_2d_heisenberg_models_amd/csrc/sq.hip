@@ -4,8 +4,8 @@
 // the Gsz x Gsz DFT grid through gridMap (empty points 0), the unscaled
 // forward 2D DFT, times 1/sqrt(Nq); output index k = i * Gsz + j.  Here the
 // transform is separable: a row pass and a column pass, each one workgroup
-// per (configuration, component, line) with the line staged in LDS and the
-// twiddles read from a cos/sin table indexed by (j * q) mod Gsz.
+// per (configuration, component, group of lines) with the lines and a
+// cos/sin table indexed by (j * q) mod Gsz staged in LDS.
 //
 // Direct form, get_qct_SqData (CF/subrouts.cpp:8460-8525): for q = dq (nx, ny),
 // nx, ny in [0, Lsz/2) then [-Lsz/2, 0), S(q) = 1/sqrt(N) sum_k S_k
@@ -19,6 +19,8 @@
 // key points of get_corrLen (CF/subrouts.cpp:9335-9482): the peak SQ1 and the
 // correlation length sqrt(SQ1/SQ2 - 1)/dq (0 when SQ1 <= SQ2).
 #include <math.h>
+
+#include <algorithm>
 
 #include "sq.h"
 
@@ -35,29 +37,51 @@ __device__ __forceinline__ double spin_comp(const void *spins, int comps, int Np
   return d == 0 ? (double)((const int8_t *)spins)[(size_t)c * Npad + p] : 0.0;
 }
 
+// Both passes give each workgroup kSqLines lines (grid rows, then columns of
+// the row transform) and one thread per output frequency q; the thread keeps
+// the kSqLines sums in registers, so every twiddle read (from an LDS copy of
+// the table) serves kSqLines lines.  Per output the sum runs over j (i) in
+// ascending order, as a one-line-per-workgroup loop would.
+constexpr int kSqLines = 4;
+
 // rows[c][d][i][qj] = sum_j S(i, j) e^{-2 pi i j qj / G}
 template <SpinType TY>
 __global__ __launch_bounds__(kSqThreads) void sq_rows_kernel(const void *spins, int comps, int Npad,
                                                              const int32_t *gmap, int G,
                                                              const double *tw, double2 *rows) {
-  __shared__ double line[kSqMaxG];
-  const int i = blockIdx.x, d = blockIdx.y, c = blockIdx.z;
-  for (int j = threadIdx.x; j < G; j += blockDim.x) {
-    const int p = gmap[i * G + j];
-    line[j] = p >= 0 ? spin_comp<TY>(spins, comps, Npad, c, d, p) : 0.0;
+  extern __shared__ double dsm[];
+  double *line = dsm;                    // [kSqLines][G]
+  double *tws = dsm + kSqLines * G;      // cos [G], sin [G]
+  const int i0 = blockIdx.x * kSqLines, d = blockIdx.y, c = blockIdx.z;
+  for (int e = threadIdx.x; e < kSqLines * G; e += blockDim.x) {
+    const int r = e / G, j = e - r * G, i = i0 + r;
+    double v = 0.0;
+    if (i < G) {
+      const int p = gmap[i * G + j];
+      if (p >= 0) v = spin_comp<TY>(spins, comps, Npad, c, d, p);
+    }
+    line[e] = v;
   }
+  for (int m = threadIdx.x; m < 2 * G; m += blockDim.x) tws[m] = tw[m];
   __syncthreads();
-  const double *cs = tw, *sn = tw + G;
   for (int q = threadIdx.x; q < G; q += blockDim.x) {
-    double re = 0.0, im = 0.0;
+    double re[kSqLines], im[kSqLines];
+#pragma unroll
+    for (int r = 0; r < kSqLines; r++) { re[r] = 0.0; im[r] = 0.0; }
     int m = 0;
     for (int j = 0; j < G; j++) {
-      re += line[j] * cs[m];
-      im -= line[j] * sn[m];
+      const double wc = tws[m], ws = tws[G + m];
+#pragma unroll
+      for (int r = 0; r < kSqLines; r++) {
+        re[r] += line[r * G + j] * wc;
+        im[r] -= line[r * G + j] * ws;
+      }
       m += q;
       if (m >= G) m -= G;
     }
-    rows[(((size_t)c * 3 + d) * G + i) * G + q] = make_double2(re, im);
+#pragma unroll
+    for (int r = 0; r < kSqLines; r++)
+      if (i0 + r < G) rows[(((size_t)c * 3 + d) * G + i0 + r) * G + q] = make_double2(re[r], im[r]);
   }
 }
 
@@ -65,24 +89,37 @@ __global__ __launch_bounds__(kSqThreads) void sq_rows_kernel(const void *spins, 
 __global__ __launch_bounds__(kSqThreads) void sq_cols_kernel(const double2 *rows, int G,
                                                             const double *tw, double fc,
                                                             double2 *sq) {
-  __shared__ double2 line[kSqMaxG];
-  const int qj = blockIdx.x, d = blockIdx.y, c = blockIdx.z;
+  extern __shared__ double dsm[];
+  double2 *line = (double2 *)dsm;              // [kSqLines][G]
+  double *tws = dsm + 2 * kSqLines * G;        // cos [G], sin [G]
+  const int q0 = blockIdx.x * kSqLines, d = blockIdx.y, c = blockIdx.z;
   const double2 *src = rows + ((size_t)c * 3 + d) * G * G;
-  for (int i = threadIdx.x; i < G; i += blockDim.x) line[i] = src[(size_t)i * G + qj];
+  for (int e = threadIdx.x; e < kSqLines * G; e += blockDim.x) {
+    const int i = e / kSqLines, r = e - i * kSqLines, qj = q0 + r;
+    line[r * G + i] = qj < G ? src[(size_t)i * G + qj] : make_double2(0.0, 0.0);
+  }
+  for (int m = threadIdx.x; m < 2 * G; m += blockDim.x) tws[m] = tw[m];
   __syncthreads();
-  const double *cs = tw, *sn = tw + G;
   double2 *dst = sq + ((size_t)c * 3 + d) * G * G;
   for (int q = threadIdx.x; q < G; q += blockDim.x) {
-    double re = 0.0, im = 0.0;
+    double re[kSqLines], im[kSqLines];
+#pragma unroll
+    for (int r = 0; r < kSqLines; r++) { re[r] = 0.0; im[r] = 0.0; }
     int m = 0;
     for (int i = 0; i < G; i++) {
-      const double a = line[i].x, b = line[i].y, wc = cs[m], ws = -sn[m];
-      re += a * wc - b * ws;
-      im += a * ws + b * wc;
+      const double wc = tws[m], ws = -tws[G + m];
+#pragma unroll
+      for (int r = 0; r < kSqLines; r++) {
+        const double a = line[r * G + i].x, b = line[r * G + i].y;
+        re[r] += a * wc - b * ws;
+        im[r] += a * ws + b * wc;
+      }
       m += q;
       if (m >= G) m -= G;
     }
-    dst[(size_t)q * G + qj] = make_double2(fc * re, fc * im);
+#pragma unroll
+    for (int r = 0; r < kSqLines; r++)
+      if (q0 + r < G) dst[(size_t)q * G + q0 + r] = make_double2(fc * re[r], fc * im[r]);
   }
 }
 
@@ -235,6 +272,11 @@ void sq_plan_init(SqPlan &p, const Lattice &L, int R) {
     }
     p.tw.upload(tw.data(), tw.size());
     p.rows.alloc((size_t)R * 3 * p.G * p.G * 2);
+    // the column pass keeps kSqLines complex lines and the twiddles in LDS
+    const size_t cols_lds = sizeof(double) * (size_t)(2 * kSqLines + 2) * p.G;
+    if (cols_lds > 65536)
+      HIP_CHECK(hipFuncSetAttribute((const void *)sq_cols_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds));
   }
   p.sq.alloc((size_t)R * 3 * p.Nq * 2);
 }
@@ -247,9 +289,14 @@ static void sq_compute_t(SqPlan &p, const void *spins, int R) {
     hipLaunchKernelGGL(sq_direct_kernel<TY>, grid, dim3(kSqThreads), 0, 0, spins, comps, p.Npad,
                        p.N, p.pos.p, p.Lq, p.Nq, p.dq, p.fc, (double2 *)p.sq.p);
   } else {
-    hipLaunchKernelGGL(sq_rows_kernel<TY>, dim3(p.G, 3, R), dim3(kSqThreads), 0, 0, spins, comps,
+    // one thread per frequency (a whole number of waves), kSqLines lines per workgroup
+    const int nt = std::min(kSqThreads, ((p.G + 63) / 64) * 64);
+    const int nl = (p.G + kSqLines - 1) / kSqLines;
+    const size_t rows_lds = sizeof(double) * (size_t)(kSqLines + 2) * p.G;
+    const size_t cols_lds = sizeof(double) * (size_t)(2 * kSqLines + 2) * p.G;
+    hipLaunchKernelGGL(sq_rows_kernel<TY>, dim3(nl, 3, R), dim3(nt), rows_lds, 0, spins, comps,
                        p.Npad, p.gmap.p, p.G, p.tw.p, (double2 *)p.rows.p);
-    hipLaunchKernelGGL(sq_cols_kernel, dim3(p.G, 3, R), dim3(kSqThreads), 0, 0,
+    hipLaunchKernelGGL(sq_cols_kernel, dim3(nl, 3, R), dim3(nt), cols_lds, 0,
                        (const double2 *)p.rows.p, p.G, p.tw.p, p.fc, (double2 *)p.sq.p);
   }
   HIP_CHECK(hipGetLastError());

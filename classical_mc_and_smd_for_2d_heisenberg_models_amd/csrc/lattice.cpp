@@ -523,11 +523,18 @@ void Lattice::rebuild() {
     colour = col;
     n_colours = nc;
   }
-  // relabel: sort by (colour, original index)
+  // relabel: sort by (colour, coupled-neighbour count descending, original
+  // index).  Sites of one colour are updated independently, so the order
+  // inside a class changes no result; on irregular graphs it gives each
+  // 64-site wave of a class neighbours counts close to one another.
+  std::vector<int> degree(N, 0);
+  for (int i = 0; i < N; i++)
+    for (int s = 0; s < Z; s++) degree[i] += tab[(size_t)i * Zt + slot_col(s)] >= 0 ? 1 : 0;
   perm.resize(N);
   for (int i = 0; i < N; i++) perm[i] = i;
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](int a, int b) { return colour[a] < colour[b]; });
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+    return colour[a] != colour[b] ? colour[a] < colour[b] : degree[a] > degree[b];
+  });
   inv.assign(N, 0);
   for (int p = 0; p < N; p++) inv[perm[p]] = p;
   col_off.assign(n_colours + 1, 0);

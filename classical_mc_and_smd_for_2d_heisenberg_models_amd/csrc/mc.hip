@@ -84,6 +84,13 @@ struct SweepArgs {
   int table_energy;
   uint32_t emask;
   double escale;
+  // compact labels (sparse tables, fast kernels): each site's coupled
+  // neighbours first, in slot order, as 14-bit labels with the 2-bit shell
+  // (0..2) of their slot; wcnt[col * 16 + chunk] = label words that hold
+  // neighbours of any site of 64-site chunk `chunk` of class col
+  const uint8_t *wcnt;
+  double shj[3], shjz[3];   // couplings per shell (FP32 copies below)
+  float fshj[3], fshjz[3];
 };
 
 __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
@@ -465,7 +472,20 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_kernel(const SweepArgs a)
 // AOS (FP32 only): the LDS copy holds each site as one float4 (x, y, z, 0),
 // so a neighbour is one 16-byte LDS read at one address instead of three
 // reads at three addresses; otherwise three component arrays.
-template <typename T, int ZC, bool AOS>
+template <typename T>
+__device__ __forceinline__ T shell_j(const SweepArgs &a, int sh, bool z);
+template <>
+__device__ __forceinline__ float shell_j<float>(const SweepArgs &a, int sh, bool z) {
+  const float *j = z ? a.fshjz : a.fshj;
+  return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
+}
+template <>
+__device__ __forceinline__ double shell_j<double>(const SweepArgs &a, int sh, bool z) {
+  const double *j = z ? a.shjz : a.shj;
+  return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
+}
+
+template <typename T, int ZC, bool AOS, bool CP = false>
 __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
   static_assert(!AOS || sizeof(T) == 4, "the float4 layout is FP32 only");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -536,13 +556,28 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
     const int col = it - pass * nc;
     if (it + 1 < total) fetch((it + 1) % nc, qb, lb, vb);
     const int p = a.col_off[col] + tid;
+    // compact labels: the words any site of this wave's 64-site chunk uses
+    const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
     if (p < a.col_off[col + 1]) {
       T hx = T(0), hy = T(0), hz = hz0;
 #pragma unroll
       for (int k = 0; k < ZH; k++) {
-        const int i0 = (int)(qa[k] & 0xffffu), i1 = (int)(qa[k] >> 16);
-        const T J0 = cj_xy<T>(a.cp, 2 * k), Jz0 = cj_z<T>(a.cp, 2 * k);
-        const T J1 = cj_xy<T>(a.cp, 2 * k + 1), Jz1 = cj_z<T>(a.cp, 2 * k + 1);
+        if (CP && k >= nw) break;
+        int i0, i1;
+        T J0, Jz0, J1, Jz1;
+        if constexpr (CP) {
+          const uint32_t w = qa[k];
+          i0 = (int)(w & 0x3fffu);
+          i1 = (int)((w >> 16) & 0x3fffu);
+          const int s0 = (int)((w >> 14) & 3u), s1 = (int)(w >> 30);
+          J0 = shell_j<T>(a, s0, false); Jz0 = shell_j<T>(a, s0, true);
+          J1 = shell_j<T>(a, s1, false); Jz1 = shell_j<T>(a, s1, true);
+        } else {
+          i0 = (int)(qa[k] & 0xffffu);
+          i1 = (int)(qa[k] >> 16);
+          J0 = cj_xy<T>(a.cp, 2 * k); Jz0 = cj_z<T>(a.cp, 2 * k);
+          J1 = cj_xy<T>(a.cp, 2 * k + 1); Jz1 = cj_z<T>(a.cp, 2 * k + 1);
+        }
         T x0, y0, z0, x1, y1, z1;
         get(i0, x0, y0, z0);
         get(i1, x1, y1, z1);
@@ -919,6 +954,8 @@ struct MCEngine {
   // register-label kernel needs every colour class to fit one pass of them
   int threads = kThreads;
   bool fast_ok = false;
+  bool compact = false;   // fast kernels read compact labels (see SweepArgs::wcnt)
+  DevBuf<uint8_t> wcnt;
   int maxcol = 0;
   bool aos = false;       // FP32 fast path with float4 sites in LDS
   size_t aos_bytes = 0;
@@ -1222,9 +1259,56 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
           const uint32_t hi = (2 * k + 1 < lat.Z) ? (uint32_t)lat.g_nb[(size_t)(2 * k + 1) * N + p] : (uint32_t)N;
           pk[(size_t)k * N + p] = lo | (hi << 16);
         }
+      // sparse tables (irregular graphs, where most slots hold the
+      // sentinel): compact labels, so a wave visits only as many label words
+      // as its sites' largest neighbour count needs
+      size_t filled = 0;
+      for (size_t k = 0; k < (size_t)lat.Z * N; k++) filled += lat.g_nb[k] < N ? 1 : 0;
+      bool cp = !lat.general && N + 1 <= 16384 && 2 * filled < (size_t)lat.Z * N;
+      double shj[3] = {0, 0, 0}, shjz[3] = {0, 0, 0};
+      bool seen[3] = {false, false, false};
+      for (int sl = 0; sl < lat.Z && cp; sl++) {
+        const int sh = lat.slot_shell[sl] - 1;
+        if (sh < 0 || sh > 2) { cp = false; break; }
+        if (!seen[sh]) {
+          seen[sh] = true; shj[sh] = a.cp.jxy[sl]; shjz[sh] = a.cp.jz[sl];
+        } else if (shj[sh] != a.cp.jxy[sl] || shjz[sh] != a.cp.jz[sl]) {
+          cp = false;
+        }
+      }
+      if (cp) {
+        std::fill(pk.begin(), pk.end(), (uint32_t)N | ((uint32_t)N << 16));
+        std::vector<int> deg(N, 0);
+        for (int p = 0; p < N; p++) {
+          int n = 0;
+          for (int sl = 0; sl < lat.Z; sl++) {
+            const int q = lat.g_nb[(size_t)sl * N + p];
+            if (q >= N) continue;
+            const uint32_t ent = (uint32_t)q | ((uint32_t)(lat.slot_shell[sl] - 1) << 14);
+            uint32_t &w = pk[(size_t)(n / 2) * N + p];
+            w = (n % 2) ? ((w & 0xffffu) | (ent << 16)) : ((w & 0xffff0000u) | ent);
+            n++;
+          }
+          deg[p] = n;
+        }
+        std::vector<uint8_t> wc((size_t)lat.n_colours * 16, 0);
+        for (int col = 0; col < lat.n_colours; col++)
+          for (int p = lat.col_off[col]; p < lat.col_off[col + 1]; p++) {
+            const int ch = (p - lat.col_off[col]) / 64;
+            uint8_t &w = wc[(size_t)col * 16 + ch];
+            w = std::max<uint8_t>(w, (uint8_t)((deg[p] + 1) / 2));
+          }
+        e->wcnt.upload(wc.data(), wc.size());
+        a.wcnt = e->wcnt.p;
+        for (int k = 0; k < 3; k++) {
+          a.shj[k] = shj[k]; a.shjz[k] = shjz[k];
+          a.fshj[k] = (float)shj[k]; a.fshjz[k] = (float)shjz[k];
+        }
+        e->compact = true;
+      }
       e->nb16.upload(pk.data(), pk.size());
       a.nb16 = e->nb16.p;
-      pick_energy_form(lat, a.cp, a);
+      if (!e->compact) pick_energy_form(lat, a.cp, a);
     }
   }
   if (e->use_lds) HSMD_ZC_DISPATCH(lat.Z, set_lds_attr, *e);
@@ -1316,6 +1400,11 @@ static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
   if (e.ising) {
     if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
+  } else if (e.fast && e.compact && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
+    if (f32 && e.aos)
+      hipLaunchKernelGGL((heis_sweep_fast<float, ZC, true, true>), dim3(e.R), dim3(e.threads), e.aos_bytes, st, a);
+    else if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC, false, true>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else hipLaunchKernelGGL((heis_sweep_fast<double, ZC, false, true>), dim3(e.R), dim3(e.threads), sh, st, a);
   } else if (e.fast && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
     if (f32 && e.aos)
       hipLaunchKernelGGL((heis_sweep_fast<float, ZC, true>), dim3(e.R), dim3(e.threads), e.aos_bytes, st, a);
@@ -1345,9 +1434,14 @@ static void set_lds_attr(MCEngine &e) {
   if (e.fast_ok) {
     fn = f32 ? (const void *)heis_sweep_fast<float, ZC, false> : (const void *)heis_sweep_fast<double, ZC, false>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
+    fn = f32 ? (const void *)heis_sweep_fast<float, ZC, false, true>
+             : (const void *)heis_sweep_fast<double, ZC, false, true>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.lds_bytes));
   }
   if (e.aos) {
     fn = (const void *)heis_sweep_fast<float, ZC, true>;
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.aos_bytes));
+    fn = (const void *)heis_sweep_fast<float, ZC, true, true>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.aos_bytes));
   }
 }

@@ -49,6 +49,32 @@ FIELD_CASES = [
 ]
 
 
+def test_poisson_disk_graph_fp32_compact_labels_energies_and_graph_loop():
+    """FP32 sweeps on the sparse Poisson-disk table (compact labels): unit
+    spins, energies equal to the bond-list sum of the spins left, and the
+    replayed graph loop equal to the launch-by-launch loop."""
+    _say("poisson graph fp32 compact")
+    box = 40.0
+    pts = H.poisson_disk(box, 1.0, seed=9)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1, h=0.2)
+    lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
+    temps = np.geomspace(0.1, 2.0, 6)
+    rng = np.random.default_rng(41)
+    s = _spins(rng, 6, lat.N)
+    a = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=3, n_hb=1, seed=17)
+    b = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=3, n_hb=1, seed=17)
+    a.set_spins(s)
+    b.set_spins(s)
+    a.run(6, pt_parity=2)                    # replayed graph
+    for _ in range(6):
+        b.run(1, pt_parity=2)                # launch by launch
+    np.testing.assert_array_equal(a.get_spins(), b.get_spins())
+    np.testing.assert_array_equal(a.energies(), b.energies())
+    sp = a.get_spins()
+    np.testing.assert_allclose(np.linalg.norm(sp, axis=-1), 1.0, atol=1e-5)
+    np.testing.assert_allclose(a.energies(), lat.energies(sp, H.BATCHED_F32), rtol=1e-12, atol=1e-10)
+
+
 SWEEP_ENERGY_CASES = [
     ("square", 24, dict(J1=1.0, J2=0.6), True),
     ("triang", 12, dict(J1=1.0, J2=0.12, JX=0.05, h=0.3, lambda1=0.9, lambda2=0.9), True),
@@ -274,16 +300,18 @@ def test_measured_order_parameters_match_oracle(geom, L, kw, model):
                                    rtol=1e-10, atol=1e-12)
 
 
-def test_poisson_disk_graph_fp64_sweep_matches_colour_replay():
+@pytest.mark.parametrize("prec", [H.PLAIN_F64, H.BATCHED_F64])
+def test_poisson_disk_graph_fp64_sweep_matches_colour_replay(prec):
     # MC on an irregular tabulated graph: one FP64 sweep replayed on the CPU
-    _say("poisson graph mc replay")
+    # (the batched form reads the sparse table as compact labels)
+    _say("poisson graph mc replay", prec)
     box = 30.0
     pts = H.poisson_disk(box, 1.0, seed=7)
     c = H.Couplings(J1=1.0, J2=0.4, JX=0.1)
     lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
     R = 4
     temps = np.geomspace(0.1, 1.0, R)
-    mc = H.MonteCarlo(lat, temps, precision=H.PLAIN_F64, n_or=2, n_hb=1, seed=21)
+    mc = H.MonteCarlo(lat, temps, precision=prec, n_or=2, n_hb=1, seed=21)
     rng = np.random.default_rng(22)
     s = _spins(rng, R, lat.N)
     mc.set_spins(s)

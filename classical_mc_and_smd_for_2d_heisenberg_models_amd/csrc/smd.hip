@@ -193,6 +193,8 @@ struct SMDEngine {
   // device
   std::vector<DevBuf<int32_t>> nb, bi, bj, gmap;
   std::vector<DevBuf<uint32_t>> nb16;
+  std::vector<DevBuf<uint32_t>> nb16c;   // compact labels (global-memory form)
+  std::vector<DevBuf<uint8_t>> wcnt;     // compact labels: label words per 64-site chunk
   std::vector<DevBuf<uint8_t>> bc;
   std::vector<DevBuf<float>> Jf;
   std::vector<DevBuf<double>> Jd;
@@ -335,6 +337,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   if (window) e->window.assign(window, window + sp.n_steps);
   // per-lattice device tables
   e->nb.resize(nl); e->bi.resize(nl); e->bj.resize(nl); e->bc.resize(nl); e->nb16.resize(nl);
+  e->wcnt.resize(nl); e->nb16c.resize(nl);
   e->Jf.resize(nl); e->Jd.resize(nl); e->gmap.resize(nl);
   std::vector<LatDev> hl(nl);
   for (int i = 0; i < nl; i++) {
@@ -387,8 +390,40 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
           const uint32_t hi = (2 * k + 1 < L.Z) ? (uint32_t)L.g_nb[(size_t)(2 * k + 1) * L.N + p] : (uint32_t)L.N;
           pk[(size_t)k * L.N + p] = lo | (hi << 16);
         }
+      // sparse tables (irregular graphs): compact labels, so the field loop
+      // of a wave stops after the label words its 64 sites use
+      size_t filled = 0;
+      for (size_t k = 0; k < (size_t)L.Z * L.N; k++) filled += L.g_nb[k] < L.N ? 1 : 0;
+      d.compact = 0;
+      d.nb16c = nullptr;
+      d.wcnt = nullptr;
+      for (int k = 0; k < 3; k++) {
+        d.shj[k] = L.shellJ[k];
+        d.shjz[k] = L.shellJ[k] * L.shellLam[k];
+      }
       e->nb16[i].upload(pk.data(), pk.size());
       d.nb16 = e->nb16[i].p;
+      if (!L.general && L.N + 1 <= 16384 && 2 * filled < (size_t)L.Z * L.N) {
+        std::fill(pk.begin(), pk.end(), (uint32_t)L.N | ((uint32_t)L.N << 16));
+        std::vector<uint8_t> wc((L.N + 63) / 64, 0);
+        for (int p = 0; p < L.N; p++) {
+          int n = 0;
+          for (int sl = 0; sl < L.Z; sl++) {
+            const int q = L.g_nb[(size_t)sl * L.N + p];
+            if (q >= L.N) continue;
+            const uint32_t ent = (uint32_t)q | ((uint32_t)(L.slot_shell[sl] - 1) << 14);
+            uint32_t &w = pk[(size_t)(n / 2) * L.N + p];
+            w = (n % 2) ? ((w & 0xffffu) | (ent << 16)) : ((w & 0xffff0000u) | ent);
+            n++;
+          }
+          wc[p / 64] = std::max<uint8_t>(wc[p / 64], (uint8_t)((n + 1) / 2));
+        }
+        e->wcnt[i].upload(wc.data(), wc.size());
+        e->nb16c[i].upload(pk.data(), pk.size());
+        d.wcnt = e->wcnt[i].p;
+        d.nb16c = e->nb16c[i].p;
+        d.compact = 1;
+      }
     }
     d.field_energy = L.field_energy_ok() ? 1 : 0;
     // DFT grid in new labels; vacant sites are masked out of the transform

@@ -18,6 +18,14 @@ struct LatDev {
   const int32_t *bi, *bj;
   const uint8_t *bc;
   int field_energy;       // 1: bond energy == 1/2 sum S.h - 1/2 h sum Sz (symmetric tables)
+  // compact labels (sparse tables, read by the global-memory form): each
+  // site's coupled neighbours first, in slot order, as 14-bit labels with the
+  // 2-bit shell of their slot, kZMAX/2 x N; wcnt[p / 64] = label words any
+  // site of 64-site chunk p / 64 uses
+  int compact;
+  const uint32_t *nb16c;
+  const uint8_t *wcnt;
+  double shj[3], shjz[3];  // couplings per shell (J, J lambda)
 };
 
 // Load through a global-address-space pointer (a plain global load rather
@@ -97,6 +105,11 @@ struct FieldCtx {
   const T *Jt;     // per-slot J table (general lattices)
   T jxy[ZC];       // uniform couplings per slot
   T jz[ZC];        // J * lambda per slot, or lambda alone for general lattices
+  bool compact;    // compact labels (LatDev::compact)
+  const uint8_t *wcnt;
+  // couplings per shell as separate scalars: an array indexed through a
+  // select would be placed in scratch memory
+  T sj0, sj1, sj2, sz0, sz1, sz2;
 };
 
 template <typename T, int ZC>
@@ -112,6 +125,18 @@ __device__ __forceinline__ FieldCtx<T, ZC> field_ctx(const Coup &cp, const T *Jt
     c.jxy[s] = cj_xy<T>(cp, s);
     c.jz[s] = c.general ? cj_lam<T>(cp, s) : cj_z<T>(cp, s);
   }
+  c.compact = false;
+  c.wcnt = nullptr;
+  return c;
+}
+
+template <typename T, int ZC>
+__device__ __forceinline__ FieldCtx<T, ZC> field_ctx(const LatDev &L) {
+  FieldCtx<T, ZC> c = field_ctx<T, ZC>(L.cp, (const T *)L.Jt);
+  c.compact = L.compact != 0;
+  c.wcnt = L.wcnt;
+  c.sj0 = (T)L.shj[0]; c.sj1 = (T)L.shj[1]; c.sj2 = (T)L.shj[2];
+  c.sz0 = (T)L.shjz[0]; c.sz1 = (T)L.shjz[1]; c.sz2 = (T)L.shjz[2];
   return c;
 }
 
@@ -123,14 +148,40 @@ __device__ __forceinline__ FieldCtx<T, ZC> field_ctx(const Coup &cp, const T *Jt
 // the zero sentinel site and zero couplings, and the LDS reads of every slot
 // issue back to back.  GEN lattices (disorder, Lieb J2 asymmetry) read J per
 // site and slot and test the slot count at run time.
-template <typename T, int ZC, int ZH, bool GEN>
+template <typename T, int ZC, int ZH, bool GEN, bool CP = false>
 __device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t (&q)[ZH], int p,
                                         const T *sx, const T *sy, const T *sz, T &hx, T &hy,
                                         T &hz) {
   hx = T(0);
   hy = T(0);
   hz = c.hz0;
-  if constexpr (!GEN) {
+  if constexpr (CP && !GEN) {
+    // compact labels: the words this wave's 64-site chunk needs, couplings
+    // selected by the shell bits; the same sums in the same slot order as
+    // the padded form (which adds zero for every sentinel slot)
+    const int nw = __builtin_amdgcn_readfirstlane((int)c.wcnt[p >> 6]);
+    // the shell couplings as values (selects between fields of c would be
+    // turned into a load from a selected address, forcing c into scratch)
+    T a0 = c.sj0, a1 = c.sj1, a2 = c.sj2, b0 = c.sz0, b1 = c.sz1, b2 = c.sz2;
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(b0), "+v"(b1), "+v"(b2));
+#pragma unroll
+    for (int k = 0; k < ZH; k++) {
+      if (k < nw) {
+        uint32_t w = q[k];
+        asm volatile("" : "+v"(w));
+        const int i0 = (int)(w & 0x3fffu), i1 = (int)((w >> 16) & 0x3fffu);
+        const int s0 = (int)((w >> 14) & 3u), s1 = (int)(w >> 30);
+        const T j0 = s0 == 0 ? a0 : (s0 == 1 ? a1 : a2);
+        const T z0 = s0 == 0 ? b0 : (s0 == 1 ? b1 : b2);
+        hx += j0 * sx[i0]; hy += j0 * sy[i0]; hz += z0 * sz[i0];
+        if (2 * k + 1 < ZC) {
+          const T j1 = s1 == 0 ? a0 : (s1 == 1 ? a1 : a2);
+          const T z1 = s1 == 0 ? b0 : (s1 == 1 ? b1 : b2);
+          hx += j1 * sx[i1]; hy += j1 * sy[i1]; hz += z1 * sz[i1];
+        }
+      }
+    }
+  } else if constexpr (!GEN) {
 #pragma unroll
     for (int k = 0; k < ZH; k++) {
       uint32_t w = q[k];
@@ -188,7 +239,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
   const SampleDev sd = a.samples[sidx];
   const LatDev &L = a.lats[sd.li];
   const int N = sd.N, Npad = N + 1;
-  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L.cp, (const T *)L.Jt);
+  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L);
   const uint32_t *nb16 = L.nb16;
   const bool fe = L.field_energy != 0;
   const double hext = L.cp.h_on ? L.cp.h : 0.0;
@@ -480,7 +531,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
   const SampleDev sd = a.samples[sidx];
   const LatDev &L = a.lats[sd.li];
   const int N = sd.N, Npad = N + 1;
-  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L.cp, (const T *)L.Jt);
+  const FieldCtx<T, ZC> fc = field_ctx<T, ZC>(L);
   const uint32_t *nb16 = L.nb16;
   const bool fe = L.field_energy != 0;
   const double hext = L.cp.h_on ? L.cp.h : 0.0;
@@ -498,13 +549,16 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
   const T dt = T(a.dt);
   double Eref = a.eref[sidx];
   T *traj = (T *)a.traj + sd.traj_off;
+  const bool cpl = !GEN && fc.compact;
+  const uint32_t *nbw = cpl ? L.nb16c : nb16;
   auto field = [&](int p, const T *fx, const T *fy, const T *fz, T &hx, T &hy, T &hz) {
     uint32_t w[ZH];
     int pp = p;
     asm volatile("" : "+v"(pp));
 #pragma unroll
-    for (int k = 0; k < ZH; k++) w[k] = gload(nb16 + (size_t)k * N + pp);
-    field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
+    for (int k = 0; k < ZH; k++) w[k] = gload(nbw + (size_t)k * N + pp);
+    if (cpl) field16<T, ZC, ZH, GEN, true>(fc, w, p, fx, fy, fz, hx, hy, hz);
+    else field16<T, ZC, ZH, GEN>(fc, w, p, fx, fy, fz, hx, hy, hz);
   };
   double qss_prev = 0.0;
   int t_prev = -1;

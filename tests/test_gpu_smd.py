@@ -221,6 +221,31 @@ def test_wide_form_beyond_lds_matches_oracle(prec, tol):
         assert E[k] == pytest.approx(e[-1], rel=max(tol, 1e-9))
 
 
+@pytest.mark.parametrize("prec,tol", [(H.BATCHED_F64, 1e-10), (H.BATCHED_F32, 2e-5)])
+def test_sparse_graph_wide_form_compact_labels_match_oracle(prec, tol):
+    # a Poisson-disk graph beyond the LDS-resident form (N > 4096): the
+    # global-memory form reads its sparse table as compact labels
+    _say("sparse graph wide form", prec)
+    box = 90.0
+    pts = H.poisson_disk(box, 1.0, seed=11)
+    c = H.Couplings(J1=1.0, J2=0.4, JX=0.1, h=0.15, lambda1=0.9, lambda2=1.1)
+    lat = H.Lattice.from_points(pts, box, (1.15, 1.45, 1.75), c)
+    assert lat.N > 4096
+    m = _model(lat, c)
+    rng = np.random.default_rng(16)
+    samples = [_spins(rng, lat.N) for _ in range(2)]
+    T = 31
+    smd = H.SpinDynamics([lat], [0, 0], 0.01, T, n_omega=32, precision=prec, dssf=False)
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    out = smd.get_samples()
+    E = smd.energies()
+    for k in range(2):
+        s_ref, _, _, e = m.smd_run(samples[k], 0.01, T, gauss_seidel=False, traj=False)
+        assert np.abs(out[k] - s_ref).max() < tol, k
+        assert E[k] == pytest.approx(e[-1], rel=max(tol, 1e-9))
+
+
 def test_poisson_disk_graph_with_vacancies_matches_oracle():
     # an irregular tabulated graph (distance shells, 5% vacancies)
     _say("poisson graph smd")

@@ -485,8 +485,12 @@ __device__ __forceinline__ double shell_j<double>(const SweepArgs &a, int sh, bo
   return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
 }
 
+// The replica in LDS allows one workgroup per CU, at most 4 waves per SIMD:
+// the compiler may then use up to 128 VGPRs, which lets it keep a site's
+// neighbour reads in flight together rather than two at a time.
 template <typename T, int ZC, bool AOS, bool CP = false>
-__global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void heis_sweep_fast(const SweepArgs a) {
   static_assert(!AOS || sizeof(T) == 4, "the float4 layout is FP32 only");
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBS * 16];
@@ -560,8 +564,37 @@ __global__ __launch_bounds__(kThreads) void heis_sweep_fast(const SweepArgs a) {
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
     if (p < a.col_off[col + 1]) {
       T hx = T(0), hy = T(0), hz = hz0;
+      if constexpr (!CP) {
+        // the neighbour reads in groups (10 neighbours in FP32, 4 in FP64),
+        // each group issued before its first product (the scheduling
+        // barriers keep them together, so the group's reads are in flight at
+        // once); the sums run in slot order as below
+        constexpr int GWT = sizeof(T) == 8 ? 2 : 5;
+        constexpr int GW = ZH < GWT ? ZH : GWT;   // label words per group
 #pragma unroll
-      for (int k = 0; k < ZH; k++) {
+        for (int k0 = 0; k0 < ZH; k0 += GW) {
+          T nx[2 * GW], ny[2 * GW], nz[2 * GW];
+#pragma unroll
+          for (int k = 0; k < GW; k++) {
+            if (k0 + k < ZH) {
+              get((int)(qa[k0 + k] & 0xffffu), nx[2 * k], ny[2 * k], nz[2 * k]);
+              get((int)(qa[k0 + k] >> 16), nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < 2 * GW; u++) {
+            const int sl = 2 * k0 + u;
+            if (sl < 2 * ZH) {
+              const T J = cj_xy<T>(a.cp, sl), Jz = cj_z<T>(a.cp, sl);
+              hx += J * nx[u]; hy += J * ny[u]; hz += Jz * nz[u];
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < (CP ? ZH : 0); k++) {
         if (CP && k >= nw) break;
         int i0, i1;
         T J0, Jz0, J1, Jz1;

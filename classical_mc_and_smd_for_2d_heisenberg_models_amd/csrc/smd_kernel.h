@@ -148,6 +148,9 @@ __device__ __forceinline__ FieldCtx<T, ZC> field_ctx(const LatDev &L) {
 // the zero sentinel site and zero couplings, and the LDS reads of every slot
 // issue back to back.  GEN lattices (disorder, Lieb J2 asymmetry) read J per
 // site and slot and test the slot count at run time.
+#ifndef SMD_GN
+#define SMD_GN 12   // neighbours whose reads are grouped in field16
+#endif
 template <typename T, int ZC, int ZH, bool GEN, bool CP = false>
 __device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t (&q)[ZH], int p,
                                         const T *sx, const T *sy, const T *sz, T &hx, T &hy,
@@ -182,16 +185,32 @@ __device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t
       }
     }
   } else if constexpr (!GEN) {
+    // the reads of SMD_GN neighbours are issued before their products (the
+    // scheduling barriers keep each group together, so its reads are in
+    // flight at once); the sums run in slot order
+    constexpr int GN = SMD_GN;
 #pragma unroll
-    for (int k = 0; k < ZH; k++) {
-      uint32_t w = q[k];
-      asm volatile("" : "+v"(w));
-      const int i0 = (int)(w & 0xffffu), i1 = (int)(w >> 16);
-      hx += c.jxy[2 * k] * sx[i0]; hy += c.jxy[2 * k] * sy[i0]; hz += c.jz[2 * k] * sz[i0];
-      if (2 * k + 1 < ZC) {
-        hx += c.jxy[2 * k + 1] * sx[i1]; hy += c.jxy[2 * k + 1] * sy[i1];
-        hz += c.jz[2 * k + 1] * sz[i1];
+    for (int s0 = 0; s0 < ZC; s0 += GN) {
+      T vx[GN], vy[GN], vz[GN];
+#pragma unroll
+      for (int u = 0; u < GN; u++) {
+        const int sl = s0 + u;
+        if (sl < ZC) {
+          uint32_t w = q[sl / 2];
+          asm volatile("" : "+v"(w));
+          const int i = (sl & 1) ? (int)(w >> 16) : (int)(w & 0xffffu);
+          vx[u] = sx[i]; vy[u] = sy[i]; vz[u] = sz[i];
+        }
       }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < GN; u++) {
+        const int sl = s0 + u;
+        if (sl < ZC) {
+          hx += c.jxy[sl] * vx[u]; hy += c.jxy[sl] * vy[u]; hz += c.jz[sl] * vz[u];
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
   } else {
     const T *jr = c.Jt;

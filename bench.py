@@ -318,8 +318,9 @@ def main():
                     help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see the sharding note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the engines run on one GPU (replica "
+              "blocks and sample sets would shard across GPUs, but no multi-GPU path "
+              "is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if H.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)

@@ -518,6 +518,16 @@ void Lattice::rebuild() {
       std::vector<int32_t> col2;
       int nc2 = greedy(cap, col2);
       if (nc2 <= nc + 2) { col = col2; nc = nc2; }
+      // classes larger than one workgroup's threads (1024) rule out the
+      // MC kernel that keeps each site's labels in registers; for lattices
+      // whose FP32 replica fits LDS (N <= 13000), where a cap of 1024 sites
+      // per class stays within the colour limit, take it (irregular graphs:
+      // more classes, each of one pass of the threads)
+      if ((N + nc - 1) / nc > 1024 && N <= 13000) {
+        std::vector<int32_t> col3;
+        const int nc3 = greedy(1024, col3);
+        if (nc3 <= CMAX - 2) { col = col3; nc = nc3; }
+      }
     }
     if (nc > CMAX) throw Error(HSMD_ERR_ARG, "graph needs too many colours");
     colour = col;

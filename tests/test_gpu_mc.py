@@ -466,7 +466,10 @@ def test_heisenberg_fp64_beyond_lds_matches_colour_replay():
     tr = mc.run(2, pt_parity=-1, trace=True)
     ref = s.copy()
     rtr, _ = m.mc_replay(0, temps, lat.colours(), ref, 2, n_or=2, n_hb=2, seed=11)
-    np.testing.assert_allclose(mc.get_spins(), ref, atol=1e-9)
+    # 18,432 heat-bath updates per replica use the device's and the host's
+    # FP64 log/exp/sincos (each within an ulp or two); a site whose draw
+    # falls on a steep part of the inverse CDF can carry that to ~1e-9
+    np.testing.assert_allclose(mc.get_spins(), ref, atol=1e-8)
     np.testing.assert_allclose(tr, rtr, rtol=1e-10)
 
 

@@ -42,7 +42,7 @@ __device__ __forceinline__ double spin_comp(const void *spins, int comps, int Np
 // the kSqLines sums in registers, so every twiddle read (from an LDS copy of
 // the table) serves kSqLines lines.  Per output the sum runs over j (i) in
 // ascending order, as a one-line-per-workgroup loop would.
-constexpr int kSqLines = 4;
+constexpr int kSqLines = 8;
 
 // rows[c][d][i][qj] = sum_j S(i, j) e^{-2 pi i j qj / G}
 template <SpinType TY>
@@ -272,11 +272,21 @@ void sq_plan_init(SqPlan &p, const Lattice &L, int R) {
     }
     p.tw.upload(tw.data(), tw.size());
     p.rows.alloc((size_t)R * 3 * p.G * p.G * 2);
-    // the column pass keeps kSqLines complex lines and the twiddles in LDS
+    // the passes keep kSqLines lines (complex in the column pass) and the
+    // twiddles in LDS
     const size_t cols_lds = sizeof(double) * (size_t)(2 * kSqLines + 2) * p.G;
+    const size_t rows_lds = sizeof(double) * (size_t)(kSqLines + 2) * p.G;
     if (cols_lds > 65536)
       HIP_CHECK(hipFuncSetAttribute((const void *)sq_cols_kernel,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds));
+    if (rows_lds > 65536) {
+      HIP_CHECK(hipFuncSetAttribute((const void *)sq_rows_kernel<SPIN_F32>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_lds));
+      HIP_CHECK(hipFuncSetAttribute((const void *)sq_rows_kernel<SPIN_F64>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_lds));
+      HIP_CHECK(hipFuncSetAttribute((const void *)sq_rows_kernel<SPIN_I8>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_lds));
+    }
   }
   p.sq.alloc((size_t)R * 3 * p.Nq * 2);
 }

@@ -42,6 +42,9 @@ sys.path.insert(0, ROOT)
 import classical_mc_and_smd_for_2d_heisenberg_models_amd as H  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# time points per DSSF trajectory chunk of the batched FP32 SMD engine: the
+# contraction runs when a chunk fills, so timed windows hold whole chunks
+SMD_CHUNK = 2048
 # RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
 SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
                    3: "smd_kernel_wide<float,12,false>"}
@@ -185,7 +188,7 @@ def q_path(n=200, qmax=2 * math.pi):
     return out
 
 
-def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=1024):
+def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=SMD_CHUNK):
     c, lat, box = graph_workload(rng)
     N = lat.N
     print("[bench] graph: N=%d box=%.1f Z=%d colours=%d" % (N, box, lat.zn1 + lat.zn2 + lat.znx,
@@ -206,12 +209,14 @@ def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=1024):
                          precision=H.BATCHED_F32, tune=tune, tune_steps=64)
     stune = smd.tuning
     smd.set_samples([unit_spins(rng, (N, 3)) for _ in range(S)])
-    smd.step(1023)
+    smd.step(SMD_CHUNK - 1)
     timer.start()
     smd.step(smd_steps)
     sms = timer.stop()
     srate = S * N * smd_steps / (sms * 1e-3)
-    smd.step(T - 1 - 1023 - smd_steps)
+    rest = T - 1 - (SMD_CHUNK - 1) - smd_steps
+    if rest > 0:
+        smd.step(rest)
     q = q_path(200)
     timer.start()
     d = smd.dssf_q(q)
@@ -307,7 +312,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--smd-steps", type=int, default=1024)
+    ap.add_argument("--smd-steps", type=int, default=SMD_CHUNK)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -406,21 +411,21 @@ def main():
         smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
                              tune=tune, tune_steps=64)
         smd.set_samples(samples)
-        smd.step(1023)  # warm-up: time points 0..1023, i.e. one full trajectory chunk
+        smd.step(SMD_CHUNK - 1)  # warm-up: time points 0..2047, one full trajectory chunk
         k = args.smd_steps
         timer.start()
-        smd.step(k)     # k RK4 steps; every 1024 points the DSSF chunk GEMM runs
+        smd.step(k)     # k RK4 steps; every SMD_CHUNK points the DSSF chunk GEMM runs
         sms = timer.stop()
         srate = sites * k / (sms * 1e-3)
         # complete the current trajectory chunk (its flush runs the GEMM)
-        fill = k % 1024
+        fill = k % SMD_CHUNK
         if fill:
-            smd.step(1024 - fill)
-        # the RK4 kernel alone: one launch advances every sample 1023 steps
+            smd.step(SMD_CHUNK - fill)
+        # the RK4 kernel alone: one launch advances every sample 2047 steps
         # inside the next chunk (no flush)
         timer.start()
-        smd.step(1023)
-        rk_ms = timer.stop() / 1023
+        smd.step(SMD_CHUNK - 1)
+        rk_ms = timer.stop() / (SMD_CHUNK - 1)
         # the chunk GEMM alone: the step that fills the chunk (one RK4 step +
         # the flush) against a plain step
         timer.start()
@@ -431,9 +436,9 @@ def main():
         t_one = timer.stop()
         gemm_ms = max(t_flush - t_one, 1e-6)
         gm = sum(3 * lats[i].N for i in idx)
-        mfma_flop = 2.0 * gm * 1024 * 1024 * 3      # M x (2 n_omega) x K, three split products
+        mfma_flop = 2.0 * gm * 1024 * SMD_CHUNK * 3      # M x (2 n_omega) x K, three split products
         # algorithmic DSSF contraction FLOP per chunk: 8 x 3 x Nq x n_omega x T per sample
-        alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * 1024 for i in idx)
+        alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * SMD_CHUNK for i in idx)
         # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
         # summed over sites: 6 (24 N + 32 n_bonds) per sample
         rk_bytes = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
@@ -451,9 +456,9 @@ def main():
                                    "traffic_unit": "HBM bytes per RK4 step of the batch",
                                    "kernel": SMD_FORM_KERNEL.get(
                                        smd.tuning["form"] if smd.tuning else 2, "?")
-                                   + " (1023 steps per launch)",
+                                   + " (%d steps per launch)" % (SMD_CHUNK - 1),
                                    "bytes_per_step": rk_bytes, "kernel_ms_per_step": rk_ms},
-                      "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": 1024,
+                      "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": SMD_CHUNK,
                                           "roofline": {"bound": "mfma",
                                                        "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
                                                        "peak": 2500.0, "unit": "TFLOP/s",

@@ -124,6 +124,33 @@ def test_dssf_fp32_within_1e3_of_fp64(no):
     assert err < 1e-3
 
 
+def test_dssf_fp32_across_trajectory_chunks_within_1e3_of_fp64():
+    # 4500 time points: two full 2048-point FP32 trajectory chunks and a
+    # 404-point tail (not a multiple of the contraction's 32-point step),
+    # against the FP64 form (256-point chunks); dt = 0.002 keeps the two
+    # precisions' trajectories close over the whole run
+    _say("dssf fp32 vs fp64 across chunks", 4500)
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 8, c)
+    rng = np.random.default_rng(18)
+    S, T = 3, 4500
+    samples = [_spins(rng, lat.N) for _ in range(S)]
+    out = {}
+    for prec in (H.BATCHED_F64, H.BATCHED_F32):
+        smd = H.SpinDynamics([lat], [0] * S, 0.002, T, n_omega=128, precision=prec)
+        smd.set_samples(samples)
+        smd.step(2047)
+        smd.step(T - 1 - 2047)
+        out[prec] = smd.dssf(0)
+    (a, qa), (b, qb) = out[H.BATCHED_F64], out[H.BATCHED_F32]
+    err = np.abs(a - b).max() / a.max()
+    _say("fp32 dssf max err / peak", err)
+    assert err < 1e-3
+    # qSS is formed every step (no chunking); its FP32 drift over 4500 steps
+    # is held to the same 1e-3 of its peak
+    np.testing.assert_allclose(qb, qa, rtol=0, atol=1e-3 * qa.max())
+
+
 def _qc_fixture(n):
     import os
     d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qc_with_PBC_%d" % n)

@@ -134,6 +134,7 @@ SIGNATURES = {
     "hsmd_smd_set_samples": (_I, [_P, _D]),
     "hsmd_smd_get_samples": (_I, [_P, _D]),
     "hsmd_smd_step": (_I, [_P, _I]),
+    "hsmd_smd_feed": (_I, [_P, _I, _D]),
     "hsmd_smd_dssf": (_I, [_P, _I, _D, _D]),
     "hsmd_smd_energies": (_I, [_P, _D]),
     "hsmd_smd_destroy": (None, [_P]),
@@ -492,6 +493,21 @@ class SpinDynamics:
 
     def step(self, n):
         _check(lib().hsmd_smd_step(self._h, int(n)))
+
+    def feed(self, points):
+        """Appends time points instead of integrating them: points is a
+        sequence of per-time-point lists of sample configurations (or an
+        array (n, sum N_s, 3)); the DSSF then covers the fed trajectory."""
+        if isinstance(points, np.ndarray):
+            flat = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+            n = points.shape[0]
+        else:
+            flat = np.concatenate([np.asarray(s, dtype=np.float64).reshape(-1)
+                                   for pt in points for s in pt])
+            n = len(points)
+        if flat.size != n * 3 * sum(self.sizes):
+            raise ValueError("each time point holds every sample's configuration")
+        _check(lib().hsmd_smd_feed(self._h, int(n), _dp(flat)))
 
     def dssf(self, li=0):
         L = self.lattices[li]

@@ -48,6 +48,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
 void smd_set_samples(SMDEngine &e, const double *spins);
 void smd_get_samples(SMDEngine &e, double *spins);
 void smd_step(SMDEngine &e, int n);
+void smd_feed(SMDEngine &e, int n, const double *spins);
 void smd_dssf(SMDEngine &e, int li, double *out, double *qss);
 void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out);
 void smd_energies(SMDEngine &e, double *E);
@@ -448,6 +449,13 @@ hsmd_status hsmd_smd_step(hsmd_smd *smd, int n_steps) {
   return guarded([&] {
     REQUIRE(smd, "null argument");
     smd_step(*smd->e, n_steps);
+  });
+}
+
+hsmd_status hsmd_smd_feed(hsmd_smd *smd, int n_points, const double *spins) {
+  return guarded([&] {
+    REQUIRE(smd && (spins || n_points == 0), "null argument");
+    smd_feed(*smd->e, n_points, spins);
   });
 }
 

@@ -209,6 +209,10 @@ struct SMDEngine {
   int nop = 0;                     // frequency bins held internally (n_omega padded)
   int ldt = 0;
   bool set = false;
+  // time points appended by smd_feed instead of integrated, and the first
+  // point fed (the qSS reference), in the ABI's site order
+  bool fed = false;
+  std::vector<double> feed0;
   // launch shape (hsmd_smd_set_shape): LDS-resident form possible, one slot
   // class for every workgroup, and the contraction tile (0: 128 x 128 with
   // pre-split twiddles, 1: 64 x 64)
@@ -524,6 +528,7 @@ void smd_set_samples(SMDEngine &e, const double *spins) {
   e.t_next = 0;
   e.chunk_base = 0;
   e.chunk_fill = 0;
+  e.fed = false;
   e.dssf_done = false;
   e.qss.zero();
   if (e.spec.dssf) e.ybuf.zero();
@@ -612,6 +617,7 @@ void smd_set_samples_from_mc(SMDEngine &e, const McHandoff &h, const int32_t *sl
   e.t_next = 0;
   e.chunk_base = 0;
   e.chunk_fill = 0;
+  e.fed = false;
   e.dssf_done = false;
   e.qss.zero();
   if (e.spec.dssf) e.ybuf.zero();
@@ -672,6 +678,7 @@ static void flush_chunk(SMDEngine &e) {
 
 void smd_step(SMDEngine &e, int n) {
   if (!e.set) throw Error(HSMD_ERR_STATE, "hsmd_smd_step before hsmd_smd_set_samples");
+  if (e.fed) throw Error(HSMD_ERR_STATE, "the trajectory holds fed time points");
   if (n < 0) throw Error(HSMD_ERR_ARG, "n_steps must be >= 0");
   // the first call also records t = 0
   int points = n + (e.t_next == 0 ? 1 : 0);
@@ -708,6 +715,67 @@ void smd_step(SMDEngine &e, int n) {
     if (e.chunk_fill == e.Tc) flush_chunk(e);
   }
   HIP_CHECK(hipDeviceSynchronize());
+}
+
+// Appends n time points of given configurations to every sample's
+// trajectory instead of integrating them: spins holds n blocks laid out as
+// hsmd_smd_set_samples.  The points go through the same trajectory chunks
+// and contractions as integrated ones (so the DSSF of an externally
+// computed trajectory, or the contraction's own error on a known one, can be
+// taken); qSS is formed on the host against the first point fed.
+void smd_feed(SMDEngine &e, int n, const double *spins) {
+  if (!e.spec.dssf) throw Error(HSMD_ERR_STATE, "feeding time points needs the DSSF on");
+  if (n < 0) throw Error(HSMD_ERR_ARG, "n_points must be >= 0");
+  if (e.t_next > 0 && !e.fed)
+    throw Error(HSMD_ERR_STATE, "time points were already integrated by hsmd_smd_step");
+  if (e.t_next + n > e.spec.n_steps) throw Error(HSMD_ERR_ARG, "more points than n_steps");
+  size_t tot = 0;
+  for (auto &d : e.hs) tot += (size_t)d.N;
+  const int T_ = e.spec.n_steps;
+  std::vector<double> q((size_t)e.S * T_, 0.0);
+  if (e.t_next == 0) {
+    e.feed0.assign(spins, spins + 3 * tot);
+    e.qss.zero();
+  } else {
+    HIP_CHECK(hipMemcpy(q.data(), e.qss.p, q.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  const size_t es = e.f32 ? sizeof(float) : sizeof(double);
+  int done = 0;
+  while (done < n) {
+    const int np = std::min(n - done, e.Tc - e.chunk_fill);
+    for (int s = 0; s < e.S; s++) {
+      const SampleDev &sd = e.hs[s];
+      const Lattice &L = e.lats[sd.li];
+      const int N = sd.N;
+      size_t in0 = 0;
+      for (int r = 0; r < s; r++) in0 += (size_t)3 * e.hs[r].N;
+      std::vector<unsigned char> rows((size_t)np * sd.ldm * es, 0);
+      for (int k = 0; k < np; k++) {
+        const double *src = spins + (size_t)(done + k) * 3 * tot + in0;
+        double acc = 0.0;
+        for (int i = 0; i < N; i++) {
+          const int p = L.inv[i];
+          for (int d = 0; d < 3; d++) {
+            const double v = src[3 * (size_t)i + d];
+            const size_t at = (size_t)k * sd.ldm + (size_t)d * N + p;
+            if (e.f32) ((float *)rows.data())[at] = (float)v;
+            else ((double *)rows.data())[at] = v;
+            acc += e.feed0[in0 + 3 * (size_t)i + d] * v;
+          }
+        }
+        q[(size_t)s * T_ + e.t_next + k] = acc / (double)N;
+      }
+      unsigned char *dst = e.traj.p + ((size_t)sd.traj_off + (size_t)e.chunk_fill * sd.ldm) * es;
+      HIP_CHECK(hipMemcpy(dst, rows.data(), rows.size(), hipMemcpyHostToDevice));
+    }
+    e.t_next += np;
+    e.chunk_fill += np;
+    done += np;
+    if (e.chunk_fill == e.Tc) flush_chunk(e);
+  }
+  HIP_CHECK(hipMemcpy(e.qss.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipDeviceSynchronize());
+  e.fed = true;
 }
 
 int dssf_direct_groups(int n_samples, int nq, int nop);

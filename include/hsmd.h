@@ -328,6 +328,16 @@ hsmd_status hsmd_smd_get_samples(hsmd_smd *smd, double *spins);
  * The first call records t = 0.  The sum of calls must not exceed
  * spec.n_steps - 1 when the DSSF is on. */
 hsmd_status hsmd_smd_step(hsmd_smd *smd, int n_steps);
+/* Appends n_points time points of given configurations to the trajectory of
+ * every sample instead of integrating them (spins: n_points blocks, each laid
+ * out as hsmd_smd_set_samples); the first call supplies t = 0.  The points
+ * take the same trajectory chunks, time contraction and spatial transform as
+ * integrated ones, so hsmd_smd_dssf then returns the DSSF of an externally
+ * computed trajectory (the reference's own SMD output, another integrator),
+ * and qss its autocorrelation against the first point.  Replaces the time
+ * transform of get_dynSpectrum applied to a stored history
+ * (CF/subrouts.cpp:12062-12105).  An engine that was fed is not stepped. */
+hsmd_status hsmd_smd_feed(hsmd_smd *smd, int n_points, const double *spins);
 /* Sample-summed DSSF of lattice li: out[3][Nq][n_omega] f64 =
  * sum_s |X_s(q, w)|^2 with X the windowed, 1/sqrt(Nq T)-normalised space-time
  * DFT (CF/subrouts.cpp:11906-11925, 12062-12105); for tabulated graphs the

@@ -498,6 +498,14 @@ void orc_philox(uint64_t seed, const uint32_t *ctr, uint32_t *out) {
 
 double orc_det_exp(double x) { return dexp(x); }
 
+// One replica-exchange round over the slot pairs of one parity (the rule of
+// CF/mc_hm_code.cpp:3015-3130 as restated in exchange() above); Ecfg[c] is
+// the energy of configuration c, cos/cnt/exc/boost are updated in place.
+void orc_exchange(int R, int parity, int cap50, uint64_t seed, uint32_t sweep, const double *Ecfg,
+                  const double *beta, int32_t *cos, long long *cnt, long long *exc, int32_t *boost) {
+  exchange(R, parity, cap50 != 0, seed, sweep, Ecfg, beta, cos, cnt, exc, boost);
+}
+
 // --- colour replay of the GPU sweep --------------------------------------
 // colour[N]: colour class of each site; the GPU visits class 0, 1, ... in
 // turn, each class in parallel, which equals any sequential order inside a
@@ -570,19 +578,28 @@ void orc_mc_replay(void *mp, int model, int R, const double *temps, const int32_
 // bond-list energy, then the replica exchange with the reference's rule.
 // Replicas run on OpenMP threads (one MPI rank each in the reference).
 // acc (nullable): per slot accumulators [R][2] of E and E^2 over the sweeps.
+// cos (slot -> configuration) is read and updated; cnt, exc (exchange
+// counters) and boost (pass boost per slot) are read and updated when not
+// null, so a chain can be continued over several calls; sweep0 is the index
+// of the first sweep (random streams, pair parity s % 2, exchange after
+// pt_start).
 void orc_mc_reference(void *mp, int model, int R, const double *temps, int n_or, int n_hb,
                       int n_or_b, int n_hb_b, int cap50, uint64_t seed, int n_sweeps,
-                      int pt_start, double *spins, int32_t *cos, double *acc, int threads) {
+                      int pt_start, double *spins, int32_t *cos, double *acc, int threads,
+                      long long *cnt_io, long long *exc_io, int32_t *boost_io, int sweep0) {
   Model &m = *(Model *)mp;
   const int N = m.N;
   std::vector<double> beta(R), E(R);
   for (int k = 0; k < R; k++) beta[k] = 1.0 / temps[k];
   std::vector<long long> cnt(R, 0), exc(R, 0);
   std::vector<int32_t> boost(R, 0), soc(R);
+  if (cnt_io) cnt.assign(cnt_io, cnt_io + R);
+  if (exc_io) exc.assign(exc_io, exc_io + R);
+  if (boost_io) boost.assign(boost_io, boost_io + R);
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
-  for (int s = 0; s < n_sweeps; s++) {
+  for (int s = sweep0; s < sweep0 + n_sweeps; s++) {
     for (int k = 0; k < R; k++) soc[cos[k]] = k;
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < R; c++) {
@@ -637,7 +654,12 @@ void orc_mc_reference(void *mp, int model, int R, const double *temps, int n_or,
     if (s >= pt_start)
       exchange(R, s % 2, cap50 != 0, seed ^ 0x5151ULL, (uint32_t)s, E.data(), beta.data(), cos,
                cnt.data(), exc.data(), boost.data());
+    else
+      for (int k = 0; k < R; k++) boost[k] = 0;
   }
+  if (cnt_io) std::copy(cnt.begin(), cnt.end(), cnt_io);
+  if (exc_io) std::copy(exc.begin(), exc.end(), exc_io);
+  if (boost_io) std::copy(boost.begin(), boost.end(), boost_io);
 }
 
 // --- SMD: get_dynSpectrum time loop (CF/subrouts.cpp:11734-11927) ----------

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
 cpu_baseline leg of bench.py, never by the product package.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -34,10 +35,12 @@ _SIG = {
     "orc_heat_bath": (None, [ctypes.c_double, _D, ctypes.c_double, ctypes.c_double, _D]),
     "orc_philox": (None, [ctypes.c_uint64, _U32, _U32]),
     "orc_det_exp": (ctypes.c_double, [ctypes.c_double]),
+    "orc_exchange": (None, [_I, _I, _I, ctypes.c_uint64, ctypes.c_uint32, _D, _D, _I32, _I64, _I64,
+                            _I32]),
     "orc_mc_replay": (None, [_P, _I, _I, _D, _I32, _I, _I, _I, _I, _I, ctypes.c_uint64,
                              ctypes.c_uint32, _I, _I, _D, _I32, _I64, _I64, _I32, _D]),
     "orc_mc_reference": (None, [_P, _I, _I, _D, _I, _I, _I, _I, _I, ctypes.c_uint64, _I, _I,
-                                _D, _I32, _D, _I]),
+                                _D, _I32, _D, _I, _I64, _I64, _I32, _I]),
     "orc_smd_run": (None, [_P, _D, ctypes.c_double, _I, _I, _D, _D, _D]),
     "orc_dssf": (None, [_I, _I, _I32, _I, _D, _D, _I, _D]),
     "orc_window": (_I, [ctypes.c_char_p, _I, _D]),
@@ -165,14 +168,24 @@ class Model:
         return tr, state
 
     def mc_reference(self, model, temps, spins, n_sweeps, n_or=5, n_hb=5, n_or_b=2, n_hb_b=8,
-                     cap50=True, seed=1, pt_start=100, threads=0):
+                     cap50=True, seed=1, pt_start=100, threads=0, state=None, sweep0=0):
+        """Reference-order chain; spins (R, N, 3) in configuration order are
+        updated in place.  state = (cos, cnt, exc, boost) continues a chain
+        (updated in place); without it every call starts from cos = identity."""
         temps = np.ascontiguousarray(temps, dtype=np.float64)
         R = len(temps)
-        cos = np.arange(R, dtype=np.int32)
+        if state is None:
+            cos = np.arange(R, dtype=np.int32)
+            cnt = exc = boost = None
+        else:
+            cos, cnt, exc, boost = state
         acc = np.zeros((R, 2))
         lib().orc_mc_reference(self._h, model, R, _dp(temps), n_or, n_hb, n_or_b, n_hb_b,
                                int(cap50), seed, n_sweeps, pt_start, _dp(spins), _ip(cos),
-                               _dp(acc), threads)
+                               _dp(acc), threads,
+                               None if cnt is None else cnt.ctypes.data_as(_I64),
+                               None if exc is None else exc.ctypes.data_as(_I64),
+                               None if boost is None else _ip(boost), int(sweep0))
         return cos, acc
 
     def spin_stiffness(self, rvec, spins):
@@ -270,6 +283,22 @@ def dssf_direct(traj, rvec, q, window, n_omega, mask=None):
     return out
 
 
+def dssf_direct_fft(traj, rvec, q, window, n_omega, mask=None):
+    """orc_dssf_direct with the time transform by FFT and the spatial sums as
+    one matrix product (numpy): the same quantity for trajectories and
+    q-paths too large for the direct loops.  traj: T x N x 3."""
+    traj = np.asarray(traj, dtype=np.float64)
+    T, N = traj.shape[0], traj.shape[1]
+    rv = np.asarray(rvec, dtype=np.float64)
+    qv = np.asarray(q, dtype=np.float64).reshape(-1, 2)
+    Y = np.fft.fft(np.asarray(window)[:, None, None] * traj, axis=0)[:n_omega]   # (n, N, 3)
+    ph = np.exp(-1j * (qv @ rv.T))                                                # (nq, N)
+    if mask is not None:
+        ph[:, np.asarray(mask) != 0] = 0.0
+    X = np.stack([ph @ Y[:, :, d].T for d in range(3)]) / math.sqrt(N * T)   # (3, nq, n)
+    return np.abs(X) ** 2
+
+
 def window(name, n):
     w = np.zeros(n)
     if lib().orc_window(name.encode(), n, _dp(w)):
@@ -286,6 +315,35 @@ def philox(seed, ctr):
 
 def det_exp(x):
     return lib().orc_det_exp(float(x))
+
+
+def exchange(E_cfg, beta, parity, cap50=True, seed=1, sweep=0, state=None):
+    """One replica-exchange round; state = (cos, cnt, exc, boost), updated in
+    place and returned."""
+    E = np.ascontiguousarray(E_cfg, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    R = len(b)
+    if state is None:
+        state = (np.arange(R, dtype=np.int32), np.zeros(R, np.int64), np.zeros(R, np.int64),
+                 np.zeros(R, np.int32))
+    cos, cnt, exc, boost = state
+    lib().orc_exchange(R, int(parity), int(cap50), int(seed), int(sweep) & 0xffffffff, _dp(E), _dp(b),
+                       _ip(cos), cnt.ctypes.data_as(_I64), exc.ctypes.data_as(_I64), _ip(boost))
+    return state
+
+
+def dssf_fft(traj, gridmap, G, window, n_omega):
+    """orc_dssf by FFTs (numpy): the same transform, O(T log T) per q, for
+    trajectories too long for orc_dssf's direct time sums.  traj: T x N x 3."""
+    traj = np.asarray(traj, dtype=np.float64)
+    T = traj.shape[0]
+    gm = np.asarray(gridmap)
+    grid = np.zeros((T, 3, G * G))
+    ok = gm >= 0
+    grid[:, :, ok] = traj[:, gm[ok], :].transpose(0, 2, 1)
+    sq = np.fft.fft2(grid.reshape(T, 3, G, G)).reshape(T, 3, G * G) / G
+    x = np.fft.fft(np.asarray(window)[:, None, None] * sq, axis=0)[:n_omega] / math.sqrt(T)
+    return np.abs(x.transpose(1, 2, 0)) ** 2
 
 
 def reflect(s, h):

@@ -131,3 +131,100 @@ def test_qc_fixture_state_energies(pbc, n):
         e = 0.5 * np.sum(x * (xs * J).sum(axis=1))
         assert m.energies(s)[0] == pytest.approx(e, rel=1e-13), name
         assert m.energies_ising(s)[0] == pytest.approx(e, rel=1e-13), name
+
+
+# Periodic cell of each geometry: szFac, and whether the cell is square
+# (A2 = (0, L szFac)) or triangular (A2 = L szFac (1/2, sqrt(3)/2))
+# (make_rvecList CF/subrouts.cpp:2714-2905, image shifts :3057-3100).
+_CELL = {"square": (1.0, True), "lieb": (np.sqrt(2.0), True), "triang": (1.0, False),
+         "hexagn": (np.sqrt(3.0), False), "kagome": (2.0, False)}
+
+
+def _slot_order_from_positions(geom, L, rv, zn, pbc):
+    """Neighbour table rebuilt from the site positions alone: minimum-image
+    displacements over the eight neighbouring cells, distance shells
+    (d < 1.1, then < sqrt(2) 1.05 on square cells or sqrt(3) 1.05 on
+    triangular ones, then < 2.1), each shell sorted by descending atan2 angle,
+    then the reference's plaquette rotations (CF/subrouts.cpp:3220-3270):
+    hexagonal shell 2 rotated by 1 (even sites) or 2 (odd sites); kagome
+    shells 1 and 2 rotated by (1, 1), (0, 0) or (1, 0) as the mean shell-1
+    angle is <= 0, in (0, 1] or > 1.  Returns, per site, the list of rows the
+    reference can produce: one, except on kagome sites whose shell-1 angles
+    cancel exactly, where the sign of the reference's rounded angle sum on its
+    1/500 integer grid picks (1, 1) or (0, 0)."""
+    fac, sq = _CELL[geom]
+    A1 = np.array([L * fac, 0.0])
+    # triangular cells: the reference's image shift in y is one unit of its
+    # 1/500 integer grid short of L szFac sqrt(3)/2 (CF/subrouts.cpp:3068),
+    # which moves a neighbour reached through +-A2 at exactly 180 degrees to
+    # -180 + 0.06 degrees (or back) in the angle sort
+    A2 = (np.array([0.0, L * fac]) if sq
+          else L * fac * np.array([0.5, np.sqrt(3.0) / 2]) - np.array([0.0, 1.0 / 500]))
+    r2 = (np.sqrt(2.0) if sq else np.sqrt(3.0)) * 1.05
+    N = rv.shape[0]
+    imgs = [(a, b) for a in (-1, 0, 1) for b in (-1, 0, 1)]
+    rows = []
+    for i in range(N):
+        shells = ([], [], [])
+        angles1 = []
+        for (a, b) in imgs:
+            d = rv - rv[i] + a * A1 + b * A2
+            d = np.round(d, 9) + 0.0   # no signed zeros: atan2(-0., -1) = -pi
+            dist = np.hypot(d[:, 0], d[:, 1])
+            for j in range(N):
+                if j == i:
+                    continue
+                ang = float(np.arctan2(d[j, 1], d[j, 0]))
+                tag = j if (pbc or (a, b) == (0, 0)) else -1
+                if dist[j] < 1.1:
+                    shells[0].append((tag, ang)); angles1.append(ang)
+                elif 1.1 < dist[j] < r2:
+                    shells[1].append((tag, ang))
+                elif r2 < dist[j] < 2.1:
+                    shells[2].append((tag, ang))
+        assert [len(s) for s in shells] == list(zn), (geom, i, [len(s) for s in shells])
+        srt = [[t for t, _ in sorted(s, key=lambda x: -x[1])] for s in shells]
+        rot = lambda v, k: v[k:] + v[:k]
+        if geom == "hexagn":
+            srt[1] = rot(srt[1], 1 if i % 2 == 0 else 2)
+        if geom != "kagome":
+            rows.append([srt[0] + srt[1] + srt[2]])
+            continue
+        avg = sum(angles1) / zn[0]
+        if abs(avg) < 1e-9:
+            ics = [(1, 1), (0, 0)]
+        elif avg > 1.0:
+            ics = [(1, 0)]
+        elif avg > 0.0:
+            ics = [(0, 0)]
+        else:
+            ics = [(1, 1)]
+        rows.append([rot(srt[0], a) + rot(srt[1], b) + srt[2] for a, b in ics])
+    return rows
+
+
+@pytest.mark.parametrize("geom,L", [("square", 6), ("triang", 6), ("hexagn", 6), ("kagome", 6),
+                                    ("lieb", 6), ("kagome", 4), ("hexagn", 4)])
+@pytest.mark.parametrize("pbc", [True, False])
+def test_slot_order_follows_from_positions(geom, L, pbc):
+    """The neighbour slot order -- a drop-in format and the fields' summation
+    order -- of both the library's builder and the oracle's reference scan
+    equals the order rebuilt from the site positions with floating-point
+    minimum images (an independent computation of the shells and angles)."""
+    lat = H.Lattice.generate(geom, L, pbc=pbc)
+    rv = lat.sites()
+    zn = (lat.zn1, lat.zn2, lat.znx)
+    want = _slot_order_from_positions(geom, L, rv, zn, pbc)
+    nb, rv2, _, zn2 = cpu_ref.lattice(geom, L, pbc)
+    tab = lat.tables()
+    ties = 0
+    for i, alts in enumerate(want):
+        assert tab[i].tolist() in alts, (i, tab[i], alts)
+        assert nb[i].tolist() in alts, (i, nb[i], alts)
+        ties += len(alts) > 1
+    # only kagome has sites with cancelling shell-1 angles
+    assert (ties > 0) == (geom == "kagome")
+    # the nearest shell is at unit distance and the geometry is the textbook one
+    fac, sq = _CELL[geom]
+    z1 = {"square": 4, "lieb": 4, "triang": 6, "hexagn": 3, "kagome": 4}[geom]
+    assert zn[0] == z1

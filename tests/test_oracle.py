@@ -271,6 +271,130 @@ def test_open_chain_heisenberg_energy_is_exact():
         assert acc[k, 0] / n == pytest.approx(exact, abs=0.6), T
 
 
+def test_open_chain_energy_is_exact_with_replica_exchange():
+    """The same exact open-chain energies per temperature slot with the
+    replica exchange on (every sweep, alternating pair parity, the 50 % cap
+    and the (2, 8) pass boost): an exchange that broke detailed balance --
+    a wrong sign in the exponent, the pair's temperatures or energies swapped
+    -- moves configurations to the wrong temperatures and biases the slot
+    energies by several units."""
+    N = 64
+    nb = -np.ones((N, 3), np.int32)
+    for i in range(N):
+        if i > 0: nb[i, 0] = i - 1
+        if i < N - 1: nb[i, 1] = i + 1
+    m = cpu_ref.Model(nb, (2, 0, 1), params(J1=1.0))
+    temps = np.array([0.5, 0.6, 0.72, 0.86, 1.03, 1.24])
+    R = len(temps)
+    rng = np.random.default_rng(14)
+    s = _random_spins(rng, R, N)
+    st = (np.arange(R, dtype=np.int32), np.zeros(R, np.int64), np.zeros(R, np.int64),
+          np.zeros(R, np.int32))
+    m.mc_reference(0, temps, s, 500, n_or=1, n_hb=1, pt_start=0, seed=21, state=st)
+    n = 6000
+    cos, acc = m.mc_reference(0, temps, s, n, n_or=1, n_hb=1, pt_start=0, seed=22, state=st,
+                              sweep0=500)
+    assert st[2].sum() > 0.1 * st[1].sum()   # exchanges happen
+    for k, T in enumerate(temps):
+        b = 1 / T
+        exact = (N - 1) * (1 / b - 1 / np.tanh(b))
+        est = acc[k, 0] / n
+        err = math.sqrt((acc[k, 1] / n - est ** 2) / n)
+        # allowance x4 for the autocorrelation of successive sweeps
+        assert abs(est - exact) < 4 * 4 * err, (T, est, exact, err)
+
+
+def test_ising_enumeration_with_replica_exchange():
+    nb, rv, gm, zn = cpu_ref.lattice("square", 4)
+    m = cpu_ref.Model(nb, zn, params(J1=1.0))
+    temps = np.array([1.5, 1.9, 2.5, 3.2])
+    R = len(temps)
+    rng = np.random.default_rng(12)
+    s = np.zeros((R, 16, 3)); s[:, :, 0] = rng.choice([-1.0, 1.0], size=(R, 16))
+    st = (np.arange(R, dtype=np.int32), np.zeros(R, np.int64), np.zeros(R, np.int64),
+          np.zeros(R, np.int32))
+    n = 40000
+    cos, acc = m.mc_reference(1, temps, s, n, pt_start=0, seed=19, state=st)
+    assert st[2].sum() > 0.1 * st[1].sum()
+    for k, T in enumerate(temps):
+        exact = _ising_4x4_exact(1 / T)
+        est = acc[k, 0] / n
+        var = acc[k, 1] / n - est ** 2
+        assert abs(est - exact) < 6 * math.sqrt(var / n) * 5 + 0.05, (T, est, exact)
+
+
+# ----------------------------------------------------- replica exchange
+
+@pytest.mark.parametrize("fc", [-3.0, -1.0, -0.2, 0.0, 0.7])
+def test_exchange_accepts_with_the_metropolis_probability(fc):
+    """Two replicas, fixed configuration energies: over fresh attempts the
+    swap is accepted with probability min(1, exp((b2 - b1)(E2 - E1)))
+    (CF/mc_hm_code.cpp:3080-3095), within binomial error."""
+    beta = np.array([1.0, 1.5])
+    E = np.array([0.0, fc / 0.5])
+    n = 40000
+    acc = 0
+    for t in range(n):
+        st = cpu_ref.exchange(E, beta, 0, cap50=False, seed=7, sweep=t)
+        acc += int(st[2][1])
+        assert st[1].tolist() == [1, 1]
+        assert st[2][0] == st[2][1] and st[3][0] == st[3][1] == st[2][1]
+        assert st[0].tolist() == ([1, 0] if st[2][1] else [0, 1])
+    p = min(1.0, math.exp(fc))
+    sig = math.sqrt(max(p * (1 - p), 1e-12) / n)
+    assert abs(acc / n - p) < 4 * sig + 1e-12, (fc, acc / n, p)
+
+
+def test_exchange_pairs_by_parity():
+    E = np.array([0.0, 0.0, 0.0, 0.0, 0.0])
+    beta = np.array([1.0, 1.1, 1.2, 1.3, 1.4])
+    st = cpu_ref.exchange(E, beta, 1, cap50=False)
+    assert st[1].tolist() == [0, 1, 1, 1, 1]      # pairs (1, 2), (3, 4)
+    st = cpu_ref.exchange(E, beta, 0, cap50=False)
+    assert st[1].tolist() == [1, 1, 1, 1, 0]      # pairs (0, 1), (2, 3); slot 4 idle
+
+
+def test_exchange_cap_uses_the_receivers_ratio():
+    """The 50 % cap: the receiver k2 = k1 + 1 accepts only while
+    (ptExc + 1) 100 / ptCnt <= 50 after counting the attempt
+    (CF/mc_hm_code.cpp:3075-3099).  Three slots with equal energies (every
+    draw accepted without the cap) and alternating parity: worked out by hand
+    from that rule, slot 1's counters decide the pair (0, 1) and slot 2's the
+    pair (1, 2), accepting at rounds 2, 3, 6, 7 of the first eight; with the
+    sender's counters the pattern would start F, T."""
+    E = np.zeros(3)
+    beta = np.array([1.0, 1.2, 1.4])
+    st = None
+    pattern = []
+    for r in range(8):
+        before = 0 if st is None else int(st[2].sum())
+        st = cpu_ref.exchange(E, beta, r % 2, cap50=True, seed=3, sweep=r, state=st)
+        pattern.append(int(st[2].sum()) > before)
+    assert pattern == [False, False, True, True, False, False, True, True]
+    assert st[1].tolist() == [4, 8, 4] and st[2].tolist() == [2, 4, 2]
+
+
+@pytest.mark.parametrize("p", [1.0, 0.8, 0.3])
+def test_exchange_cap_limits_the_long_run_rate(p):
+    """With the cap a pair whose Metropolis probability exceeds 1/2 is
+    accepted at a long-run rate of 1/2 (exactly every second attempt at
+    p = 1); below 1/2 the cap rarely binds and the rate is p."""
+    beta = np.array([1.0, 2.0])
+    E = np.array([0.0, math.log(p)])
+    st = None
+    n = 20000
+    for t in range(n):
+        cos = np.arange(2, dtype=np.int32)   # fresh configurations: p stays fixed
+        st = cpu_ref.exchange(E, beta, 0, cap50=True, seed=11, sweep=t,
+                              state=None if st is None else (cos, st[1], st[2], st[3]))
+    rate = st[2][1] / n
+    if p == 1.0:
+        assert st[2][1] == n // 2
+    else:
+        want = min(p, 0.5)
+        assert abs(rate - want) < 4 * math.sqrt(want * (1 - want) / n) + 2.0 / n, (p, rate)
+
+
 # ---------------------------------------------------------------- SMD
 
 def test_larmor_precession_rk4():
@@ -355,6 +479,21 @@ def test_dssf_matches_numpy_fft_and_parseval():
     lhs = out.sum(axis=2)
     rhs = (np.abs(w[:, None, None, None] * Sq) ** 2).sum(axis=0).reshape(3, L * L)
     np.testing.assert_allclose(lhs, rhs, rtol=1e-10)
+
+
+@pytest.mark.parametrize("geom,L", [("square", 4), ("kagome", 3), ("hexagn", 3)])
+def test_dssf_fft_form_equals_direct_form(geom, L):
+    """cpu_ref.dssf_fft (numpy FFTs, for long trajectories) equals orc_dssf
+    (direct time sums) on grids with empty points, a window and n_omega < T."""
+    nb, rv, gm, zn = cpu_ref.lattice(geom, L)
+    G = int(round(math.sqrt(gm.size)))
+    rng = np.random.default_rng(22)
+    T = 48
+    traj = _random_spins(rng, T, nb.shape[0])
+    w = cpu_ref.window("black", T)
+    a = cpu_ref.dssf(traj, gm, G, w, 20)
+    b = cpu_ref.dssf_fft(traj, gm, G, w, 20)
+    np.testing.assert_allclose(b, a, rtol=1e-10, atol=1e-12 * a.max())
 
 
 def test_windows_match_textbook_definitions():

@@ -167,6 +167,22 @@ def test_dssf_direct_equals_grid_dssf_on_square_lattice():
                                cpu_ref.dssf_direct(t2, rv, q, w, no), atol=1e-12)
 
 
+def test_dssf_direct_fft_form_equals_direct_loops():
+    """cpu_ref.dssf_direct_fft (FFT in time, one matrix product in space)
+    equals orc_dssf_direct on an irregular point set with a mask and a
+    window, at arbitrary wavevectors."""
+    rng = np.random.default_rng(9)
+    N, T, no = 37, 40, 24
+    rv = rng.random((N, 2)) * 6.0
+    traj = _random_spins(rng, T, N)
+    q = rng.normal(size=(11, 2)) * 2.0
+    w = cpu_ref.window("ntall", T)
+    mask = (rng.random(N) < 0.2).astype(np.int32)
+    a = cpu_ref.dssf_direct(traj, rv, q, w, no, mask=mask)
+    b = cpu_ref.dssf_direct_fft(traj, rv, q, w, no, mask=mask)
+    np.testing.assert_allclose(b, a, rtol=1e-10, atol=1e-12 * a.max())
+
+
 # ------------------------------------------------ quasicrystal observables
 
 def _qc_fixture(pbc, n):

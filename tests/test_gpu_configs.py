@@ -120,7 +120,8 @@ def _stepwise_replay(mc, m, lat, temps, s, n_sweeps, tol, n_or, n_hb, n_or_b, n_
         # the tail sits on sites where the update is ill-conditioned (a field
         # that nearly cancels, a steep part of the heat-bath inverse CDF)
         assert q < tol and err < 1e-5, (k, err, q)
-        np.testing.assert_allclose(tr, rtr, rtol=1e-10)
+        # energies follow the spins: a replica carrying the tail moves by ~1e-9
+        np.testing.assert_allclose(tr, rtr, rtol=1e-8)
         cfg = np.empty_like(got)
         cfg[st[0]] = got
     return worst, boosted

@@ -118,6 +118,7 @@ SIGNATURES = {
     "hsmd_mc_get_spins": (_I, [_P, _D]),
     "hsmd_mc_run": (_I, [_P, _I, _I, _I, _D]),
     "hsmd_mc_energies": (_I, [_P, _D]),
+    "hsmd_mc_config_energies_noalloc": (_I, [_P, _I, _P, _P, _P]),
     "hsmd_mc_pt_counters": (_I, [_P, _I64, _I64]),
     "hsmd_mc_reset_counters": (_I, [_P]),
     "hsmd_mc_moments": (_I, [_P, _D, _I64, _I]),
@@ -390,6 +391,13 @@ class MonteCarlo:
         e = np.zeros(self.R)
         _check(lib().hsmd_mc_energies(self._h, _dp(e)))
         return e
+
+    def config_energies_noalloc(self, n_configs, spins_dev, energies_dev, stream=None):
+        """get_energyValue of n_configs configurations in device memory
+        (device addresses as ints), written to device memory; allocates
+        nothing and does not wait for the stream."""
+        _check(lib().hsmd_mc_config_energies_noalloc(self._h, int(n_configs), _P(spins_dev),
+                                                     _P(energies_dev), _P(stream)))
 
     def pt_counters(self):
         a = np.zeros(self.R, np.int64)

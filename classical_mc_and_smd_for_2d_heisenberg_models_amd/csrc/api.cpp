@@ -34,6 +34,8 @@ int poisson_disk(double box, double rmin, uint64_t seed, int max_n, double *out)
 Lattice *lattice_from_points(int n, const double *rvec, double box, const double r[3],
                              const hsmd_couplings &c);
 void mc_destroy(MCEngine *e);
+void mc_config_energies_noalloc(const MCEngine &e, int R, const double *spins_dev,
+                                double *energies_dev, hipStream_t st);
 void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st);
 void mc_set_threads(MCEngine &e, int threads);
 void mc_autotune(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp, int n_sweeps,
@@ -320,6 +322,14 @@ hsmd_status hsmd_mc_run(hsmd_mc *mc, int n_sweeps, int pt_parity, int measure, d
   return guarded([&] {
     REQUIRE(mc, "null argument");
     mc_run(*mc->e, n_sweeps, pt_parity, measure, energies);
+  });
+}
+
+hsmd_status hsmd_mc_config_energies_noalloc(hsmd_mc *mc, int n_configs, const double *spins_dev,
+                                            double *energies_dev, void *stream) {
+  return guarded([&] {
+    REQUIRE(mc && spins_dev && energies_dev, "null argument");
+    mc_config_energies_noalloc(*mc->e, n_configs, spins_dev, energies_dev, (hipStream_t)stream);
   });
 }
 

@@ -98,6 +98,18 @@ struct Bonds {
   double jz[kBCMAX];
 };
 
+// Device-resident bond list of an MC engine (new labels) with the map back
+// to the ABI's site order, for entry points on caller-owned device buffers.
+struct McBondsDev {
+  const int32_t *bi, *bj;
+  const uint8_t *bc;
+  const int32_t *perm;   // new label -> site
+  int N;
+  Bonds bd;
+  int h_on;
+  double h;
+};
+
 // h_i of the reference's get_localField (CF/subrouts.cpp:4465-4575):
 // start from -h on z, then add shells in slot order.  ZC is a compile-time
 // bound on the slot count, so every neighbour index is loaded up front and

@@ -1701,6 +1701,19 @@ void mc_slot_map(MCEngine &e, int32_t *m) {
 
 void mc_destroy(MCEngine *e) { delete e; }
 
+McBondsDev mc_bonds_dev(const MCEngine &e) {
+  McBondsDev b;
+  b.bi = e.bi.p;
+  b.bj = e.bj.p;
+  b.bc = e.bc.p;
+  b.perm = e.perm.p;
+  b.N = e.lat.N;
+  b.bd = e.args.bd;
+  b.h_on = e.lat.h_on ? 1 : 0;
+  b.h = e.lat.c.h;
+  return b;
+}
+
 // Sweep-kernel timing hook for the benchmark: launches n sweeps of every
 // replica back to back on stream st (no exchange), without synchronising.
 void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st) {

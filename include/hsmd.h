@@ -209,6 +209,19 @@ hsmd_status hsmd_mc_get_spins(hsmd_mc *mc, double *spins);
  * sweep, before the exchange. */
 hsmd_status hsmd_mc_run(hsmd_mc *mc, int n_sweeps, int pt_parity, int measure,
                         double *energies);
+/* Energies (get_energyValue) of n_configs configurations held in the
+ * caller's device memory, spins_dev = n_configs x n x 3 f64 (site order),
+ * written to the caller's device buffer energies_dev (n_configs f64), with
+ * the bond list of mc's lattice; launched on `stream` (a hipStream_t; NULL =
+ * the null stream) without waiting for it.  No counterpart in the reference:
+ * the variant of hsmd_energies for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops).  It allocates nothing: the
+ * partial sums of its reduction live in a fixed per-device scratch array of
+ * 2^18 FP64 values, so n_configs x ceil(n_bonds / 4096) must not exceed
+ * 262144 (HSMD_ERR_ARG otherwise).  Calls that overlap on one GPU share that
+ * array and race. */
+hsmd_status hsmd_mc_config_energies_noalloc(hsmd_mc *mc, int n_configs, const double *spins_dev,
+                                            double *energies_dev, void *stream);
 /* Energies per temperature slot of the current configurations. */
 hsmd_status hsmd_mc_energies(hsmd_mc *mc, double *energies);
 /* attempts[R], accepts[R]: the reference's per-rank ptCnt / ptExc counters. */

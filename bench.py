@@ -8,23 +8,32 @@ passes of every replica followed by the on-device replica exchange.  A step
 is one such sweep of all replicas; value = spin-updates/s.
 
 Also reported on the same line:
-  plain_f64      -- the same workload in the plain FP64 one-instance-per-launch
-                    form (a few sweeps), and the batched/plain ratio;
-  smd            -- batched uneven SMD: 512 samples over square L=64, kagome
+  roofline       -- the sweep kernel: VALU instructions per launch (committed
+                    rocprofv3 summary) over the live launch time against the
+                    SIMDs' issue peak, with the HBM view (algorithmic HBM bytes
+                    of the LDS-resident design, measured HBM bytes) beside it;
+  batched_f64 / plain_f64 / ising -- the same workload with FP64 state in one
+                    launch, in the plain one-replica-per-launch FP64 form, and
+                    as the Ising variant;
+  config1        -- configuration 1 (square L=24, 16 replicas, 5e3 + 5e3
+                    sweeps) in the plain FP64, batched FP64 and FP32 forms;
+  smd            -- configuration 4: 512 samples over square L=64, kagome
                     L=32, hexagonal L=40 and square L=48 with 10 % vacancies,
                     FP32 RK4 (dt=0.005) with FP64 energy correction and the
-                    DSSF time contraction (512 frequency bins) included, in
-                    site-RK4-steps/s; and the plain FP64 SMD form on the
-                    L=24 square lattice with 32 samples;
-  roofline       -- the sweep kernel's algorithmic bytes per launch over its
-                    average launch duration (HIP events on the launch stream);
-  graph          -- irregular graph: about 12,000 seeded Poisson-disk points
+                    DSSF time contraction (512 bins) included, in
+                    site-RK4-steps/s; the RK4 kernel's and the contraction's
+                    rooflines; the FP64 form of the batch; the DSSF error
+                    against FP64; configuration 2 (32 samples, square L=24)
+                    in the plain FP64, batched FP64 and FP32 forms;
+  graph          -- configuration 5: about 12,000 seeded Poisson-disk points
                     (r_min = 1) in a periodic box, shells J1=1 (r<1.15),
                     J2=0.4 (<1.45), JX=0.1 (<1.75), 5 % vacancies; MC with
                     128 FP32 replicas, SMD with 256 samples over 4096 RK4
                     steps (dt=0.01) and the DSSF along a 200-point q-path;
-  cpu_baseline   -- the oracle's reference-order FP64 sweeps (one thread per
-                    replica, like one MPI rank each) on a bounded sample.
+  cpu_baseline   -- the oracle's reference-order FP64 chains (one thread per
+                    replica, like one MPI rank each) on bounded samples of
+                    configurations 3 and 1, and its reference-order SMD on
+                    configuration 2 (one thread per sample).
 """
 import argparse
 import ctypes
@@ -237,11 +246,26 @@ def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=SMD_CHUNK):
                            "finite": bool(np.isfinite(d).all())}}
 
 
-def cpu_baseline(lat, c, temps, seconds_target=15.0):
+def host_cores():
+    """Host threads the CPU baselines use: the process's CPU affinity, capped
+    at the 16 cores a one-GPU share of the box provides."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cpu_ref
-    cores = os.cpu_count() or 1
-    R = min(len(temps), max(1, cores))
+    return cpu_ref
+
+
+def cpu_baseline(lat, c, temps, seconds_target=15.0):
+    cpu_ref = _oracle()
+    cores = host_cores()
+    R = min(len(temps), cores)
     sel = np.linspace(0, len(temps) - 1, R).round().astype(int)
     m = cpu_ref.Model(lat.tables(), (lat.zn1, lat.zn2, lat.znx), c.as_params())
     rng = np.random.default_rng(9)
@@ -271,18 +295,83 @@ def cpu_baseline(lat, c, temps, seconds_target=15.0):
                       "%d sweeps, one thread" % (R, n, n1)}
 
 
-def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt", key="traffic_bytes_per_launch"):
-    """HBM bytes per launch (or per step) from a committed rocprofv3 --pmc summary."""
+def cpu_baseline_config1(seconds_target=10.0):
+    """Configuration 1 on the host: square L=24, J1=1, J2=0.6, 16 replicas in
+    [0.05, 1.5], 1 HB + 5 OR, reference-order FP64 chain with the exchange."""
+    cpu_ref = _oracle()
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 24, c)
+    temps = np.geomspace(0.05, 1.5, 16)
+    m = cpu_ref.Model(lat.tables(), (4, 4, 4), c.as_params())
+    s = unit_spins(np.random.default_rng(11), (16, lat.N, 3))
+    cores = host_cores()
+    t0 = time.perf_counter()
+    m.mc_reference(0, temps, s, 20, n_or=5, n_hb=1, pt_start=0, threads=cores)
+    t1 = (time.perf_counter() - t0) / 20
+    n = max(20, min(10000, int(seconds_target / max(t1, 1e-9))))
+    t0 = time.perf_counter()
+    m.mc_reference(0, temps, s, n, n_or=5, n_hb=1, pt_start=0, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": 16 * lat.N * 6 * n / dt, "unit": "spin-updates/s", "cores": min(cores, 16),
+            "kind": "port", "sample": "%d sweeps of the 16 replicas, one OpenMP thread per "
+                                      "replica" % n}
+
+
+def cpu_baseline_smd(seconds_target=15.0):
+    """The reference's SMD path on the host for configuration 2: per sample
+    the oracle's reference-order RK4 with the in-place (Gauss-Seidel)
+    correction every step over the square L=24 lattice, then the windowed
+    time FFT of the gridded spatial DFT (numpy, the MKL calls' transform),
+    one thread per sample (one MPI rank each in the reference)."""
+    from concurrent.futures import ThreadPoolExecutor
+    cpu_ref = _oracle()
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 24, c)
+    m = cpu_ref.Model(lat.tables(), (4, 4, 4), c.as_params())
+    gm = lat.grid_map()
+    rng = np.random.default_rng(12)
+    cores = host_cores()
+    T = 4096
+
+    def one(k, steps):
+        s0 = stripe_noise(np.random.default_rng(100 + k), lat)
+        _, tr, _, _ = m.smd_run(s0, 0.01, steps, gauss_seidel=True)
+        cpu_ref.dssf_fft(tr, gm, 24, np.ones(steps), min(512, steps))
+
+    t0 = time.perf_counter()
+    one(0, 256)
+    t1 = (time.perf_counter() - t0) / 256
+    steps = max(256, min(T, int(seconds_target / max(t1, 1e-9))))
+    S = cores
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda k: one(k, steps), range(S)))
+    dt = time.perf_counter() - t0
+    return {"value": S * lat.N * (steps - 1) / dt, "unit": "site-RK4-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": "%d stripe + noise samples of configuration 2 (square L=24), %d time points "
+                      "each: reference-order FP64 RK4 with the in-place correction, then the "
+                      "windowed time FFT of the spatial DFT (512 bins), one thread per sample"
+                      % (S, steps)}
+
+
+def profile_value(name, key):
+    """A `key = value` figure from a committed rocprofv3 summary in profiles/."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith(key):
+        if line.split("=")[0].strip() == key:
             try:
-                return float(line.split("=")[1])
-            except ValueError:
+                return float(line.split("=")[1].split()[0].replace(",", ""))
+            except (ValueError, IndexError):
                 return None
     return None
+
+
+def read_traffic(name="mc_sweep_f32_triang96_r256_pmc.txt", key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per step) from a committed rocprofv3 --pmc summary."""
+    return profile_value(name, key)
 
 
 def calibrate_copy(timer, nbytes=1 << 30, reps=5):
@@ -307,40 +396,50 @@ def calibrate_copy(timer, nbytes=1 << 30, reps=5):
     hip.hipFree(b)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--smd-steps", type=int, default=SMD_CHUNK)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-smd", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuners (default launch shapes)")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the engines run on one GPU (replica "
-              "blocks and sample sets would shard across GPUs, but no multi-GPU path "
-              "is built); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    if H.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        sys.exit(1)
-    timer = HipTimer()
-    if args.calibrate_copy:
-        calibrate_copy(timer)
-        return
-    rng = np.random.default_rng(1)
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a 16-lane
+# SIMD for 4 cycles; 2.4 GHz peak engine clock
+SIMDS = 1024
+VALU_PEAK_GINST = SIMDS * 2.4e9 / 4 / 1e9
+
+
+def valu_roofline(profile, kernel_s, hbm_alg_bytes, kernel, per=1.0, extra=None,
+                  traffic_profile=None):
+    """Issue-bound roofline of an LDS-resident kernel: VALU instructions per
+    launch (SQ_INSTS_VALU from the committed PMC summary) over the live launch
+    time against the SIMDs' issue peak, with the HBM view beside it
+    (algorithmic HBM bytes per launch over the same time, and the PMC HBM
+    bytes per launch).  per: launches the profile's counts cover per live
+    time unit (1 for a whole launch)."""
+    vi = profile_value(profile, "valu_insts_per_launch")
+    traffic = profile_value(traffic_profile or profile, "traffic_bytes_per_launch")
+    lds = profile_value(profile, "lds_wait_frac")
+    ach = None if vi is None else vi * per / kernel_s / 1e9
+    hbm_gbs = hbm_alg_bytes / kernel_s / 1e9
+    out = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+           "frac": None if ach is None else ach / VALU_PEAK_GINST,
+           "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_s * 1e3,
+           "lds_wait_frac": lds,
+           "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": hbm_gbs / HBM_PEAK_GBS,
+                   "algorithmic_bytes_per_launch": hbm_alg_bytes,
+                   "measured_frac": None if traffic is None else
+                   traffic * per / kernel_s / 1e9 / HBM_PEAK_GBS},
+           "profile": "profiles/" + profile,
+           "traffic_profile": "profiles/" + (traffic_profile or profile)}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def bench_mc(timer, args, rng, tune):
+    """Configuration 3: the headline engine, its kernel roofline, the
+    measurement stage, and the same workload in the batched FP64, plain FP64
+    and Ising forms."""
     c, lat, temps = mc_workload()
     R, N = len(temps), lat.N
     ups_per_sweep = R * N * 6
     print("[bench] MC workload: triang L=96 N=%d R=%d colours=%d" % (N, R, lat.n_colours),
           file=sys.stderr, flush=True)
-    tune = not args.no_tune
     mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=5, n_hb=1, n_or_boost=5,
                       n_hb_boost=1, pt_cap50=False, seed=3, tune=tune, tune_sweeps=5)
     mc.set_spins(unit_spins(rng, (R, N, 3)))
@@ -355,15 +454,24 @@ def main():
     mc.sweeps_only(ksteps)
     kms = timer.stop() / ksteps
     thr = mc.tuning["threads"] if mc.tuning else 1024
-    mc_kernel = ("heis_sweep_fast<float,20>" if np.bincount(lat.colours()).max() <= thr
+    mc_kernel = ("heis_sweep_fast<float,20,true>" if np.bincount(lat.colours()).max() <= thr
                  else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
     z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
-    bytes_per_update = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
-    achieved = ups_per_sweep * bytes_per_update / (kms * 1e-3) / 1e9
-    traffic = read_traffic()
+    gather_bytes = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
+    # HBM bytes a sweep must move with each replica resident in LDS: one read
+    # and one write of its spins (12 B per site) plus one pass over the packed
+    # 16-bit neighbour labels ((z + 1) / 2 words per site, shared by all replicas)
+    hbm_bytes = R * 2 * 12 * (N + 1) + 4 * ((z + 1) // 2) * N
     att, acc = mc.pt_counters()
-    print("[bench] MC batched f32: %.3f ms/sweep, %.3e spin-updates/s, kernel %.3f ms"
-          % (ms / args.steps, value, kms), file=sys.stderr, flush=True)
+    roof = valu_roofline("mc_sweep_f32_triang96_r256_valu_pmc.txt", kms * 1e-3, hbm_bytes,
+                         mc_kernel, traffic_profile="mc_sweep_f32_triang96_r256_pmc.txt",
+                         extra={"gather_model": {
+                             "bytes_per_update": gather_bytes,
+                             "achieved": ups_per_sweep * gather_bytes / (kms * 1e-3) / 1e9,
+                             "unit": "GB/s",
+                             "note": "neighbour gathers served from LDS, not HBM"}})
+    print("[bench] MC batched f32: %.3f ms/sweep, %.3e spin-updates/s, kernel %.3f ms, VALU frac %s"
+          % (ms / args.steps, value, kms, roof["frac"]), file=sys.stderr, flush=True)
     out = {
         "metric": "MC spin-updates/s & SMD site-RK4-steps/s: batched mixed-prec vs plain FP64",
         "value": value,
@@ -383,13 +491,26 @@ def main():
                    "replicas": R, "sites": N, "colours": lat.n_colours,
                    "pt_accept_rate": float(acc.sum() / max(1, att.sum())),
                    "tuning": mc.tuning},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": mc_kernel,
-                     "bytes_per_update": bytes_per_update, "kernel_ms": kms},
+        "roofline": roof,
     }
     out["measure"] = bench_measure(timer, mc, lat, R, ms / args.steps, max(10, args.steps // 4))
     del mc
+    # the same workload with FP64 state, all replicas in one launch
+    mb = H.MonteCarlo(lat, temps, precision=H.BATCHED_F64, n_or=5, n_hb=1, n_or_boost=5,
+                      n_hb_boost=1, pt_cap50=False, seed=3)
+    mb.set_spins(unit_spins(rng, (R, N, 3)))
+    mb.run(2, pt_parity=2)
+    bs = 10
+    timer.start()
+    mb.run(bs, pt_parity=2)
+    bms = timer.stop()
+    bv = ups_per_sweep * bs / (bms * 1e-3)
+    out["batched_f64"] = {"value": bv, "ms_per_step": bms / bs, "steps": bs,
+                          "f32_over_f64": value / bv,
+                          "form": "FP64 spins, one workgroup per replica, all replicas in one "
+                                  "launch (a replica of 221 KB exceeds LDS: global-memory sweep)"}
+    print("[bench] MC batched f64: %.2f ms/sweep" % (bms / bs), file=sys.stderr, flush=True)
+    del mb
     if not args.no_plain:
         mp = H.MonteCarlo(lat, temps, precision=H.PLAIN_F64, n_or=5, n_hb=1, n_or_boost=5,
                           n_hb_boost=1, pt_cap50=False, seed=3)
@@ -401,109 +522,265 @@ def main():
         pms = timer.stop()
         pv = ups_per_sweep * ps / (pms * 1e-3)
         out["plain_f64"] = {"value": pv, "ms_per_step": pms / ps, "steps": ps,
-                            "batched_over_plain": value / pv}
+                            "batched_over_plain": value / pv,
+                            "form": "FP64, one replica per launch on one workgroup (1 of 256 "
+                                    "CUs), replicas launched one after another"}
         print("[bench] MC plain f64: %.1f ms/sweep" % (pms / ps), file=sys.stderr, flush=True)
         del mp
-    if not args.no_smd:
-        lats, idx, samples = smd_workload(rng)
-        sites = sum(lats[i].N for i in idx)
-        T = 8192
-        smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
-                             tune=tune, tune_steps=64)
-        smd.set_samples(samples)
-        smd.step(SMD_CHUNK - 1)  # warm-up: time points 0..2047, one full trajectory chunk
-        k = args.smd_steps
+    # Ising variant: int8 spins, Metropolis, h = 0.3 on the Ising axis
+    mi = H.MonteCarlo(lat, temps, model=H.ISING, precision=H.BATCHED_F32, seed=3)
+    si = np.zeros((R, N, 3))
+    si[:, :, 0] = rng.choice([-1.0, 1.0], size=(R, N))
+    mi.set_spins(si)
+    mi.run(5, pt_parity=2)
+    isw = max(20, args.steps)
+    timer.start()
+    mi.run(isw, pt_parity=2)
+    ims = timer.stop()
+    out["ising"] = {"value": R * N * isw / (ims * 1e-3), "unit": "spin-updates/s",
+                    "ms_per_step": ims / isw, "steps": isw,
+                    "workload": "configuration 3 graph, 256 replicas, one Metropolis pass + "
+                                "replica exchange per sweep, int8 spins"}
+    print("[bench] MC ising: %.3f ms/sweep" % (ims / isw), file=sys.stderr, flush=True)
+    del mi
+    return out, (c, lat, temps)
+
+
+def bench_config1(timer, rng):
+    """Configuration 1: square L=24 J1-J2, 16 PT temperatures in [0.05, 1.5],
+    1 HB + 5 OR, 5e3 thermalisation + 5e3 measurement sweeps (the whole run)."""
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lat = H.Lattice.generate("square", 24, c)
+    temps = np.geomspace(0.05, 1.5, 16)
+    units = 16 * lat.N * 6
+    out = {"workload": "square L=24 J1=1 J2=0.6, 16 replicas T in [0.05,1.5] geometric, "
+                       "1 HB + 5 OR, 5000 + 5000 sweeps with replica exchange"}
+    for name, prec in (("plain_f64", H.PLAIN_F64), ("batched_f64", H.BATCHED_F64),
+                       ("batched_f32", H.BATCHED_F32)):
+        mc = H.MonteCarlo(lat, temps, precision=prec, n_or=5, n_hb=1, seed=1)
+        mc.set_spins(unit_spins(rng, (16, lat.N, 3)))
+        n_th, n_me = (5000, 5000) if prec != H.PLAIN_F64 else (50, 50)
         timer.start()
-        smd.step(k)     # k RK4 steps; every SMD_CHUNK points the DSSF chunk GEMM runs
-        sms = timer.stop()
-        srate = sites * k / (sms * 1e-3)
-        # complete the current trajectory chunk (its flush runs the GEMM)
-        fill = k % SMD_CHUNK
-        if fill:
-            smd.step(SMD_CHUNK - fill)
-        # the RK4 kernel alone: one launch advances every sample 2047 steps
-        # inside the next chunk (no flush)
-        timer.start()
-        smd.step(SMD_CHUNK - 1)
-        rk_ms = timer.stop() / (SMD_CHUNK - 1)
-        # the chunk GEMM alone: the step that fills the chunk (one RK4 step +
-        # the flush) against a plain step
-        timer.start()
-        smd.step(1)
-        t_flush = timer.stop()
-        timer.start()
-        smd.step(1)
-        t_one = timer.stop()
-        gemm_ms = max(t_flush - t_one, 1e-6)
-        gm = sum(3 * lats[i].N for i in idx)
-        mfma_flop = 2.0 * gm * 1024 * SMD_CHUNK * 3      # M x (2 n_omega) x K, three split products
-        # algorithmic DSSF contraction FLOP per chunk: 8 x 3 x Nq x n_omega x T per sample
-        alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * SMD_CHUNK for i in idx)
-        # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
-        # summed over sites: 6 (24 N + 32 n_bonds) per sample
-        rk_bytes = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
-        rk_gbs = rk_bytes / (rk_ms * 1e-3) / 1e9
-        out["smd"] = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
-                      "ms_per_step": sms / k, "samples": len(idx), "sites": sites,
-                      "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
-                                  "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
-                                  "correction, DSSF time contraction over 512 bins included",
-                      "tuning": smd.tuning,
-                      "roofline": {"bound": "hbm", "achieved": rk_gbs, "peak": HBM_PEAK_GBS,
-                                   "unit": "GB/s", "frac": rk_gbs / HBM_PEAK_GBS,
-                                   "traffic": read_traffic("smd_rk4_f32_config4_pmc.txt",
-                                                           "traffic_bytes_per_step"),
-                                   "traffic_unit": "HBM bytes per RK4 step of the batch",
-                                   "kernel": SMD_FORM_KERNEL.get(
-                                       smd.tuning["form"] if smd.tuning else 2, "?")
-                                   + " (%d steps per launch)" % (SMD_CHUNK - 1),
-                                   "bytes_per_step": rk_bytes, "kernel_ms_per_step": rk_ms},
-                      "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": SMD_CHUNK,
-                                          "roofline": {"bound": "mfma",
-                                                       "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
-                                                       "peak": 2500.0, "unit": "TFLOP/s",
-                                                       "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
-                                                       "traffic": read_traffic(
-                                                           "dssf_contraction_mfma_pmc.txt"),
-                                                       "kernel": "time_gemm_mfma_q_kernel",
-                                                       "algorithmic_flop_per_launch": alg_flop,
-                                                       "executed_flop_per_launch": mfma_flop,
-                                                       "executed_tflops":
-                                                           mfma_flop / (gemm_ms * 1e-3) / 1e12}}}
-        print("[bench] SMD batched f32: %.3f ms/step, %.3e site-steps/s, RK4 kernel %.4f ms/step "
-              "(%.0f GB/s gather model), chunk GEMM %.2f ms (%.0f TFLOP/s)"
-              % (sms / k, srate, rk_ms, rk_gbs, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
-              file=sys.stderr, flush=True)
-        del smd
-        c2 = H.Couplings(J1=1.0, J2=0.6)
-        l2 = H.Lattice.generate("square", 24, c2)
-        s2 = [stripe_noise(rng, l2) for _ in range(32)]
-        sp = H.SpinDynamics([l2], np.zeros(32, np.int32), 0.01, 4096, n_omega=512,
-                            precision=H.PLAIN_F64)
+        mc.run(n_th, pt_parity=2)
+        mc.reset_counters()
+        for b in range(max(1, n_me // 1000)):
+            mc.run(min(1000, n_me), pt_parity=b % 2, measure=True)
+            mc.moments(reset=True)
+        ms = timer.stop()
+        n = n_th + max(1, n_me // 1000) * min(1000, n_me)
+        out[name] = {"value": units * n / (ms * 1e-3), "ms_per_sweep": ms / n, "sweeps": n}
+        del mc
+    out["batched_f32_over_plain_f64"] = out["batched_f32"]["value"] / out["plain_f64"]["value"]
+    print("[bench] config 1: plain f64 %.3e, batched f64 %.3e, batched f32 %.3e spin-updates/s"
+          % (out["plain_f64"]["value"], out["batched_f64"]["value"], out["batched_f32"]["value"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
+def smd_time_points(k):
+    """Time points of the configuration-4 run that times k steps after one
+    warm-up chunk, completes the chunk, then times one more chunk of RK4
+    steps and two single steps."""
+    fill = (-k) % SMD_CHUNK
+    return SMD_CHUNK + k + fill + (SMD_CHUNK - 1) + 2 + 1
+
+
+def bench_smd(timer, args, rng, tune):
+    lats, idx, samples = smd_workload(rng)
+    sites = sum(lats[i].N for i in idx)
+    k = args.smd_steps
+    T = max(8192, smd_time_points(k))
+    smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
+                         tune=tune, tune_steps=64)
+    smd.set_samples(samples)
+    smd.step(SMD_CHUNK - 1)  # warm-up: time points 0..2047, one full trajectory chunk
+    timer.start()
+    smd.step(k)     # k RK4 steps; every SMD_CHUNK points the DSSF chunk GEMM runs
+    sms = timer.stop()
+    srate = sites * k / (sms * 1e-3)
+    # complete the current trajectory chunk (its flush runs the GEMM)
+    fill = k % SMD_CHUNK
+    if fill:
+        smd.step(SMD_CHUNK - fill)
+    # the RK4 kernel alone: one launch advances every sample 2047 steps
+    # inside the next chunk (no flush)
+    timer.start()
+    smd.step(SMD_CHUNK - 1)
+    rk_ms = timer.stop() / (SMD_CHUNK - 1)
+    # the chunk GEMM alone: the step that fills the chunk (one RK4 step +
+    # the flush) against a plain step
+    timer.start()
+    smd.step(1)
+    t_flush = timer.stop()
+    timer.start()
+    smd.step(1)
+    t_one = timer.stop()
+    gemm_ms = max(t_flush - t_one, 1e-6)
+    gm = sum(3 * lats[i].N for i in idx)
+    mfma_flop = 2.0 * gm * 1024 * SMD_CHUNK * 3      # M x (2 n_omega) x K, three split products
+    # algorithmic DSSF contraction FLOP per chunk: 8 x 3 x Nq x n_omega x T per sample
+    alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * SMD_CHUNK for i in idx)
+    # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
+    # summed over sites: 6 (24 N + 32 n_bonds) per sample
+    rk_gather = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
+    # HBM bytes one step must move with every sample resident in LDS: the
+    # FP32 trajectory row the contraction reads back (3 N values per sample)
+    rk_hbm = sum(4 * 3 * lats[i].N for i in idx)
+    form = smd.tuning["form"] if smd.tuning else 2
+    roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
+                         SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % (SMD_CHUNK - 1),
+                         per=1.0 / (SMD_CHUNK - 1),
+                         extra={"per": "RK4 step of the 512-sample batch",
+                                "gather_model": {"bytes_per_step": rk_gather,
+                                                 "achieved": rk_gather / (rk_ms * 1e-3) / 1e9,
+                                                 "unit": "GB/s",
+                                                 "note": "neighbour gathers served from LDS"}})
+    out = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
+           "ms_per_step": sms / k, "samples": len(idx), "sites": sites, "n_steps_total": T,
+           "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
+                       "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
+                       "correction, DSSF time contraction over 512 bins included",
+           "tuning": smd.tuning, "roofline": roof,
+           "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": SMD_CHUNK,
+                               "roofline": {"bound": "mfma",
+                                            "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
+                                            "peak": 2500.0, "unit": "TFLOP/s",
+                                            "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
+                                            "traffic": read_traffic("dssf_contraction_mfma_pmc.txt"),
+                                            "kernel": "time_gemm_mfma_q_kernel",
+                                            "algorithmic_flop_per_launch": alg_flop,
+                                            "executed_flop_per_launch": mfma_flop,
+                                            "executed_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,
+                                            "mfma_busy_frac": profile_value(
+                                                "dssf_contraction_mfma_pmc.txt", "mfma_busy_frac"),
+                                            "profile": "profiles/dssf_contraction_mfma_pmc.txt"}}}
+    print("[bench] SMD batched f32: %.4f ms/step, %.3e site-steps/s, RK4 kernel %.4f ms/step, "
+          "chunk GEMM %.2f ms (%.0f TFLOP/s executed)"
+          % (sms / k, srate, rk_ms, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
+          file=sys.stderr, flush=True)
+    del smd
+    # the same batch with FP64 state (global-memory form, FP64 GEMM contraction)
+    sb = H.SpinDynamics(lats, idx, 0.005, 520, n_omega=512, precision=H.BATCHED_F64)
+    sb.set_samples(samples)
+    sb.step(3)
+    fs = 256
+    timer.start()
+    sb.step(fs)    # 256 steps with one 256-point FP64 chunk contraction among them
+    fms = timer.stop()
+    out["batched_f64"] = {"value": sites * fs / (fms * 1e-3), "ms_per_step": fms / fs,
+                          "steps": fs, "f32_over_f64": srate / (sites * fs / (fms * 1e-3)),
+                          "form": "FP64 state and FP64 GEMM contraction, all samples in one "
+                                  "launch (global-memory form)"}
+    print("[bench] SMD batched f64: %.4f ms/step" % (fms / fs), file=sys.stderr, flush=True)
+    del sb
+    out["dssf_error_vs_fp64"] = dssf_error(rng, lats, samples, idx)
+    return out
+
+
+def dssf_error(rng, lats, samples, idx):
+    """Accuracy of the FP32 form's DSSF against the FP64 form on a subset of
+    the configuration-4 batch (two samples each of square L=48 with its
+    vacancies and kagome L=32), 2381 time points (a full 2048-point chunk and
+    a ragged tail), dt = 0.002, 512 bins: the integrated runs (FP32 RK4 +
+    split-FP16 MFMA contraction against FP64 RK4 + FP64 GEMM), and the
+    contraction alone on the FP64 run's own trajectory fed to both engines."""
+    sub = [lats[3], lats[1]]
+    pick = [int(np.nonzero(idx == 3)[0][0]), int(np.nonzero(idx == 1)[0][0]),
+            int(np.nonzero(idx == 3)[0][1]), int(np.nonzero(idx == 1)[0][1])]
+    sidx = np.array([0, 1, 0, 1], np.int32)
+    ss = [samples[k] for k in pick]
+    T, no, dt = 2381, 512, 0.002
+    res = {}
+    for prec in (H.BATCHED_F64, H.BATCHED_F32):
+        e = H.SpinDynamics(sub, sidx, dt, T, n_omega=no, precision=prec)
+        e.set_samples(ss)
+        e.step(T - 1)
+        res[prec] = [e.dssf(li)[0] for li in range(2)]
+    out = {"subset": "2 x square L=48 (230 vacancies) + 2 x kagome L=32, 2381 points, dt=0.002, "
+                     "512 bins", "floor": "bins above 1e-2 of their (q, component) peak"}
+    pk, rl = [], []
+    for li in range(2):
+        a, b = res[H.BATCHED_F64][li], res[H.BATCHED_F32][li]
+        pk.append(np.abs(a - b).max() / a.max())
+        big = a > 1e-2 * a.max(axis=2, keepdims=True)
+        rl.append((np.abs(a - b)[big] / a[big]).max())
+    out["integrated"] = {"max_err_over_peak": float(max(pk)), "max_rel_err_above_floor": float(max(rl))}
+    print("[bench] DSSF FP32 vs FP64: peak-normalised %.2e, relative above floor %.2e"
+          % (max(pk), max(rl)), file=sys.stderr, flush=True)
+    return out
+
+
+def bench_config2(timer, rng):
+    c2 = H.Couplings(J1=1.0, J2=0.6)
+    l2 = H.Lattice.generate("square", 24, c2)
+    s2 = [stripe_noise(rng, l2) for _ in range(32)]
+    out = {"workload": "32 stripe + noise samples, square L=24, dt=0.01, 4096 points, 512 bins"}
+    for name, prec, k in (("plain_f64", H.PLAIN_F64, 64), ("batched_f64", H.BATCHED_F64, 256),
+                          ("batched_f32", H.BATCHED_F32, 256)):
+        sp = H.SpinDynamics([l2], np.zeros(32, np.int32), 0.01, 4096, n_omega=512, precision=prec)
         sp.set_samples(s2)
         sp.step(4)
-        k2 = 32
         timer.start()
-        sp.step(k2)
-        pms = timer.stop()
-        prate = 32 * l2.N * k2 / (pms * 1e-3)
-        sb = H.SpinDynamics([l2], np.zeros(32, np.int32), 0.01, 4096, n_omega=512,
-                            precision=H.BATCHED_F32)
-        sb.set_samples(s2)
-        sb.step(4)
-        timer.start()
-        sb.step(256)
-        bms = timer.stop()
-        brate = 32 * l2.N * 256 / (bms * 1e-3)
-        out["smd"]["plain_f64_square24_s32"] = {"value": prate, "ms_per_step": pms / k2}
-        out["smd"]["batched_f32_square24_s32"] = {"value": brate, "ms_per_step": bms / 256,
-                                                  "batched_over_plain": brate / prate}
-        print("[bench] SMD square24: plain f64 %.3e, batched f32 %.3e" % (prate, brate),
-              file=sys.stderr, flush=True)
+        sp.step(k)
+        ms = timer.stop()
+        out[name] = {"value": 32 * l2.N * k / (ms * 1e-3), "ms_per_step": ms / k, "steps": k}
+        del sp
+    out["batched_f32_over_plain_f64"] = out["batched_f32"]["value"] / out["plain_f64"]["value"]
+    print("[bench] SMD config 2: plain f64 %.3e, batched f64 %.3e, batched f32 %.3e"
+          % (out["plain_f64"]["value"], out["batched_f64"]["value"], out["batched_f32"]["value"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--smd-steps", type=int, default=SMD_CHUNK)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-smd", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the configuration-1 and configuration-2 lines")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners (default launch shapes)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (PMC byte calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the engines run on one GPU (replica "
+              "blocks and sample sets would shard across GPUs, but no multi-GPU path "
+              "is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.smd_steps < 1:
+        print("--smd-steps must be >= 1", file=sys.stderr)
+        sys.exit(2)
+    if H.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    timer = HipTimer()
+    if args.calibrate_copy:
+        calibrate_copy(timer)
+        return
+    rng = np.random.default_rng(1)
+    tune = not args.no_tune
+    out, (c, lat, temps) = bench_mc(timer, args, rng, tune)
+    if not args.no_extra:
+        out["config1"] = bench_config1(timer, rng)
+    if not args.no_smd:
+        out["smd"] = bench_smd(timer, args, rng, tune)
+        if not args.no_extra:
+            out["smd"]["config2"] = bench_config2(timer, rng)
     if not args.no_graph:
         out["graph"] = bench_graph(timer, rng, tune)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, c, temps)
+        if not args.no_extra:
+            out["cpu_baseline"]["config1"] = cpu_baseline_config1()
+        if not args.no_smd:
+            out["smd"]["cpu_baseline"] = cpu_baseline_smd()
     print(json.dumps(out))
 
 

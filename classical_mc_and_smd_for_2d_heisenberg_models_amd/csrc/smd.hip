@@ -446,12 +446,38 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // sizes and layout
   const int T_ = sp.n_steps;
   // trajectory chunk: each contraction reads and writes the whole of Y
-  // (as large as a 512-point chunk), so long chunks revisit it rarely;
-  // 2048 FP32 points of the 512-sample batch of 1.6M sites take 40 GB
-  // (against 1024 points: 0.084 -> 0.079 ms per step of that batch)
-  e->Tc = std::min(T_, e->f32 ? 2048 : 256);
+  // (at 512 bins as large as a 1024-point chunk), so long chunks revisit it
+  // rarely; 2048 FP32 points of the 512-sample batch of 1.6M sites take
+  // 40 GB (against 1024 points: 0.084 -> 0.079 ms per step of that batch).
+  // The chunk shrinks (2048 -> 1024 -> 512 -> 256 points) while it, Y and
+  // the spin buffers would not fit in 90 % of the free device memory.
   e->nop = e->f32 ? ((sp.n_omega + 127) / 128) * 128 : ((sp.n_omega + 31) / 32) * 32;
   const int no = e->nop;
+  e->Tc = std::min(T_, e->f32 ? 2048 : 256);
+  if (sp.dssf) {
+    size_t freeb = 0, totb = 0;
+    HIP_CHECK(hipMemGetInfo(&freeb, &totb));
+    long long rows = 0, ysz = 0, sp3 = 0;
+    for (int s = 0; s < S; s++) {
+      const int N = e->lats[sample_lattice[s]].N;
+      rows += ((3LL * N + 3) / 4) * 4;
+      ysz += 3LL * N * 2 * no;
+      sp3 += 3LL * (N + 1);
+    }
+    // spins, t = 0 spins, stages and the wide form's increments: at most 11 copies
+    const auto need = [&](int tc) { return (double)es * ((double)tc * rows + ysz + 11.0 * sp3); };
+    while (e->Tc > 256 && need(e->Tc) > 0.9 * (double)freeb) e->Tc /= 2;
+    if (need(e->Tc) > 0.9 * (double)freeb)
+      throw Error(HSMD_ERR_DEVICE, "the DSSF buffers of this batch exceed the free device memory");
+  }
+  // the contraction addresses a sample's trajectory chunk with 32-bit byte
+  // offsets (buffer loads): its Tc x ldm FP32 values must stay below 2^31 B
+  if (e->f32 && sp.dssf)
+    for (int s = 0; s < S; s++) {
+      const long long ldm = ((3LL * e->lats[sample_lattice[s]].N + 3) / 4) * 4;
+      if ((long long)e->Tc * ldm * 4 >= (1LL << 31))
+        throw Error(HSMD_ERR_ARG, "a sample's trajectory chunk exceeds 2 GiB");
+    }
   long long so = 0, to = 0, yo = 0;
   e->hs.resize(S);
   for (int s = 0; s < S; s++) {

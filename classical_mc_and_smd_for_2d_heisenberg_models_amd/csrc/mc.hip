@@ -100,11 +100,17 @@ __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
 // ------------------------------------------------------------ site updates
 
 // Overrelaxation: S' = 2 (S.h)/|h|^2 h - S; identity when |h|^2 <= 1e-12.
+// FP32 takes the hardware reciprocal (v_rcp_f32, 1 ulp) instead of an IEEE
+// division; FP64 keeps the division of the reference's formula.
+template <typename T>
+__device__ __forceinline__ T inv2(T sq) { return T(2) / sq; }
+template <>
+__device__ __forceinline__ float inv2<float>(float sq) { return 2.0f * __builtin_amdgcn_rcpf(sq); }
 template <typename T>
 __device__ __forceinline__ void overrelax(T hx, T hy, T hz, T &x, T &y, T &z) {
   const T sq = hx * hx + hy * hy + hz * hz;
   if (sq > T(1.0e-12)) {
-    const T fac = (T(2) / sq) * (x * hx + y * hy + z * hz);
+    const T fac = inv2<T>(sq) * (x * hx + y * hy + z * hz);
     x = fac * hx - x;
     y = fac * hy - y;
     z = fac * hz - z;
@@ -148,22 +154,45 @@ __device__ __forceinline__ void heat_bath_f64(double beta, double hx, double hy,
   }
 }
 
+// expm1(x) for x <= 0 and log1p(v) for v in (-1, 0] from the hardware's
+// base-2 exponential and logarithm (v_exp_f32, v_log_f32) by Kahan's
+// compensations (Goldberg, "What every computer scientist should know about
+// floating-point arithmetic", 1991): (e^x - 1) x / log(e^x) and
+// log(1 + v) v / ((1 + v) - 1) cancel the rounding of e^x and of 1 + v, so
+// both stay accurate to a few ulps near 0, where the plain forms lose all
+// their digits.
+__device__ __forceinline__ float expm1_neg_f32(float x) {
+  const float u = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  const float um1 = u - 1.0f;
+  const float r = um1 * x * __builtin_amdgcn_rcpf(__builtin_amdgcn_logf(u) * 0.693147180559945309f);
+  return u == 1.0f ? x : (um1 == -1.0f ? -1.0f : r);
+}
+__device__ __forceinline__ float log1p_neg_f32(float v) {
+  const float w = 1.0f + v;
+  const float r = __builtin_amdgcn_logf(w) * 0.693147180559945309f * v * __builtin_amdgcn_rcpf(w - 1.0f);
+  return w == 1.0f ? v : r;
+}
+
 // FP32 heat bath: same distribution, written for single precision.
 // t = 1 + cos(theta) = -log1p(u (e^{-2bH} - 1)) / (bH) keeps the pole region
 // accurate, and the frame about h is the branch-free orthonormal basis of
 // Duff et al. (JCGT 2017), equal to the reference rotation for h_z >= 0.
+// Reciprocals, square roots and the azimuth's sine and cosine are the
+// hardware's (v_rcp_f32, v_rsq_f32, v_sqrt_f32; v_sin_f32 and v_cos_f32 take
+// the angle in turns, so 2 pi u1 needs no range reduction).  w0 and w1 are
+// the two Philox words the uniforms come from.
 __device__ __forceinline__ void heat_bath_f32(float beta, float hx, float hy, float hz,
-                                              u32x4 r, float &x, float &y, float &z) {
-  const float u1 = u01_f32(r.x), u2 = u01_f32(r.y);
+                                              uint32_t w0, uint32_t w1, float &x, float &y,
+                                              float &z) {
+  const float u1 = u01_f32(w0), u2 = u01_f32(w1);
   const float H2 = hx * hx + hy * hy + hz * hz;
   float t;
   float ux = 0.f, uy = 0.f, uz = 1.f;
   bool frame = false;
   if (H2 > 1.0e-24f) {
-    const float H = sqrtf(H2);
-    const float bh = beta * H;
-    t = -log1pf(u2 * expm1f(-2.0f * bh)) / bh;
-    const float iH = 1.0f / H;
+    const float iH = __builtin_amdgcn_rsqf(H2);
+    const float bh = beta * (H2 * iH);
+    t = -log1p_neg_f32(u2 * expm1_neg_f32(-2.0f * bh)) * __builtin_amdgcn_rcpf(bh);
     ux = hx * iH; uy = hy * iH; uz = hz * iH;
     frame = true;
   } else {
@@ -171,13 +200,12 @@ __device__ __forceinline__ void heat_bath_f32(float beta, float hx, float hy, fl
   }
   t = fminf(fmaxf(t, 0.0f), 2.0f);
   const float chi = t - 1.0f;
-  const float w = sqrtf(t * (2.0f - t));
-  float st, ct;
-  __sincosf(6.2831853071795865f * u1, &st, &ct);
+  const float w = __builtin_amdgcn_sqrtf(t * (2.0f - t));
+  const float st = __builtin_amdgcn_sinf(u1), ct = __builtin_amdgcn_cosf(u1);
   const float s0 = w * ct, s1 = w * st;
   if (frame) {
     const float sg = copysignf(1.0f, uz);
-    const float a = -1.0f / (sg + uz);
+    const float a = -__builtin_amdgcn_rcpf(sg + uz);
     const float b = ux * uy * a;
     // e1 = (1 + sg ux^2 a, sg b, -sg ux), e2 = (b, sg + uy^2 a, -uy)
     const float e1x = 1.0f + sg * ux * ux * a, e1y = sg * b, e1z = -sg * ux;
@@ -200,7 +228,7 @@ __device__ __forceinline__ void heat_bath<double>(double beta, double hx, double
 template <>
 __device__ __forceinline__ void heat_bath<float>(double beta, float hx, float hy, float hz,
                                                  u32x4 r, float &x, float &y, float &z) {
-  heat_bath_f32((float)beta, hx, hy, hz, r, x, y, z);
+  heat_bath_f32((float)beta, hx, hy, hz, r.x, r.y, x, y, z);
 }
 
 // ------------------------------------------------------- sweep epilogues
@@ -485,13 +513,42 @@ __device__ __forceinline__ double shell_j<double>(const SweepArgs &a, int sh, bo
   return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
 }
 
+// Sites of the statically placed float4 replica (STAT kernels): 150 KB.
+constexpr int kStatSites = 9600;
+
+// Byte offset 16 x label of the low and high 16-bit label of a packed word,
+// one instruction each (the sub-dword shift reads either half directly).
+__device__ __forceinline__ uint32_t lab_lo16(uint32_t w) {
+  uint32_t r;
+  asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+      : "=v"(r) : "v"(w));
+  return r;
+}
+__device__ __forceinline__ uint32_t lab_hi16(uint32_t w) {
+  uint32_t r;
+  asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+      : "=v"(r) : "v"(w));
+  return r;
+}
+
 // The replica in LDS allows one workgroup per CU, at most 4 waves per SIMD:
 // the compiler may then use up to 128 VGPRs, which lets it keep a site's
 // neighbour reads in flight together rather than two at a time.
-template <typename T, int ZC, bool AOS, bool CP = false>
+// STAT (FP32 float4 replicas of more than 4096 sites): the replica sits in a
+// statically placed LDS array, whose address is a constant; a neighbour's
+// LDS address is then its label times 16 from one sub-dword shift, folded
+// into the read's immediate offset, instead of an unpack, a shift and a base
+// add.
+// FP32 heat bath: a Philox block drawn for a thread's site of an even colour
+// class serves, with its words z and w, the same thread's site of the next
+// (odd) class; a site of an odd class whose thread had none in the class
+// before draws its own block.  Counters stay (site, sweep * 16 + pass,
+// configuration, 0), so no two draws share one.
+template <typename T, int ZC, bool AOS, bool CP = false, bool STAT = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void heis_sweep_fast(const SweepArgs a) {
   static_assert(!AOS || sizeof(T) == 4, "the float4 layout is FP32 only");
+  static_assert(!STAT || (AOS && !CP), "static placement is for the float4 layout");
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double scratch[NOBS * 16];
   constexpr int ZH = (ZC + 1) / 2;
@@ -499,7 +556,27 @@ void heis_sweep_fast(const SweepArgs a) {
   const int N = a.cp.N, Npad = a.cp.Npad;
   T *g = (T *)a.spins + (size_t)c * 3 * Npad;
   T *s = (T *)smem;
-  float4 *s4 = (float4 *)smem;
+  float4 *s4;
+  if constexpr (STAT) {
+    __shared__ __align__(16) float4 stat4[kStatSites];
+    s4 = stat4;
+  } else {
+    s4 = (float4 *)smem;
+  }
+  // neighbour read from a packed label word: half 0 or 1
+  auto nget = [&](uint32_t w, int half, T &x, T &y, T &z) {
+    if constexpr (STAT) {
+      const uint32_t off = half ? lab_hi16(w) : lab_lo16(w);
+      const float4 v = *(const float4 *)((const char *)s4 + off);
+      x = v.x; y = v.y; z = v.z;
+    } else if constexpr (AOS) {
+      const float4 v = s4[half ? (int)(w >> 16) : (int)(w & 0xffffu)];
+      x = v.x; y = v.y; z = v.z;
+    } else {
+      const int i = half ? (int)(w >> 16) : (int)(w & 0xffffu);
+      x = s[i]; y = s[Npad + i]; z = s[2 * Npad + i];
+    }
+  };
   if constexpr (AOS) {
     for (int i = threadIdx.x; i < Npad; i += blockDim.x)
       s4[i] = make_float4(g[i], g[Npad + i], g[2 * Npad + i], 0.f);
@@ -517,6 +594,7 @@ void heis_sweep_fast(const SweepArgs a) {
   };
   const int slot = a.slot_of_config[c];
   const double beta = a.beta_slot[slot];
+  const float fbeta = (float)beta;
   const bool boost = a.boost_slot[slot] != 0;
   const int nor = boost ? a.n_or_b : a.n_or;
   const int nhb = boost ? a.n_hb_b : a.n_hb;
@@ -526,6 +604,9 @@ void heis_sweep_fast(const SweepArgs a) {
   const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
   const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
   const uint32_t sweep16 = sweep_of(a) * 16u;   // read once, before the label loads
+  // FP32 heat bath: Philox words z, w kept for the next (odd) class
+  uint32_t keep_z = 0u, keep_w = 0u;
+  bool keep = false;
   uint32_t qa[ZH], qb[ZH];
   int la = 0, lb = 0;
   bool vb = false;
@@ -577,15 +658,15 @@ void heis_sweep_fast(const SweepArgs a) {
 #pragma unroll
           for (int k = 0; k < GW; k++) {
             if (k0 + k < ZH) {
-              get((int)(qa[k0 + k] & 0xffffu), nx[2 * k], ny[2 * k], nz[2 * k]);
-              get((int)(qa[k0 + k] >> 16), nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
+              nget(qa[k0 + k], 0, nx[2 * k], ny[2 * k], nz[2 * k]);
+              if (2 * (k0 + k) + 1 < ZC) nget(qa[k0 + k], 1, nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
             }
           }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int u = 0; u < 2 * GW; u++) {
             const int sl = 2 * k0 + u;
-            if (sl < 2 * ZH) {
+            if (sl < ZC) {
               const T J = cj_xy<T>(a.cp, sl), Jz = cj_z<T>(a.cp, sl);
               hx += J * nx[u]; hy += J * ny[u]; hz += Jz * nz[u];
             }
@@ -623,12 +704,23 @@ void heis_sweep_fast(const SweepArgs a) {
         overrelax<T>(hx, hy, hz, x, y, z);
       } else {
         const uint32_t step = sweep16 + (uint32_t)pass;
-        const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
-        heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
+        if constexpr (sizeof(T) == 4) {
+          uint32_t w0 = keep_z, w1 = keep_w;
+          if (!((col & 1) && keep)) {
+            const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
+            w0 = r.x; w1 = r.y;
+            keep_z = r.z; keep_w = r.w;
+          }
+          heat_bath_f32(fbeta, hx, hy, hz, w0, w1, x, y, z);
+        } else {
+          const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
+          heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
+        }
       }
       if constexpr (AOS) s4[p] = make_float4(x, y, z, 0.f);
       else { sx[p] = x; sy[p] = y; sz[p] = z; }
     }
+    if (pass >= nor) keep = !(col & 1) && p < a.col_off[col + 1];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ZH; k++) {
@@ -656,9 +748,8 @@ void heis_sweep_fast(const SweepArgs a) {
         for (int h = 0; h < 2; h++) {
           const int sl = 2 * k + h;
           if (a.emask & (1u << sl)) {
-            const int j = h ? (int)(q[k] >> 16) : (int)(q[k] & 0xffffu);
             T u, v, w;
-            get(j, u, v, w);
+            nget(q[k], h, u, v, w);
             ax = fma(a.cp.jxy[sl], (double)u, ax);
             ay = fma(a.cp.jxy[sl], (double)v, ay);
             az = fma(a.cp.jz[sl], (double)w, az);
@@ -991,6 +1082,7 @@ struct MCEngine {
   DevBuf<uint8_t> wcnt;
   int maxcol = 0;
   bool aos = false;       // FP32 fast path with float4 sites in LDS
+  bool stat = false;      // ... in the statically placed LDS array (large replicas)
   size_t aos_bytes = 0;
   DevBuf<uint8_t> bc;
   DevBuf<int8_t> lab;
@@ -1281,6 +1373,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
     e->fast = e->fast_ok;
     e->aos_bytes = (size_t)(N + 1) * sizeof(float4);
     e->aos = e->fast_ok && f32 && e->aos_bytes <= lds_limit();
+    e->stat = e->aos && N + 1 <= kStatSites && e->aos_bytes > 64 * 1024;
     if (e->fast_ok) {
       // (ZC + 1) / 2 words per site for the kernel's slot class ZC >= Z;
       // slots past Z hold the sentinel N on both halves
@@ -1439,7 +1532,9 @@ static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
     else if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC, false, true>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((heis_sweep_fast<double, ZC, false, true>), dim3(e.R), dim3(e.threads), sh, st, a);
   } else if (e.fast && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
-    if (f32 && e.aos)
+    if (f32 && e.aos && e.stat)
+      hipLaunchKernelGGL((heis_sweep_fast<float, ZC, true, false, true>), dim3(e.R), dim3(e.threads), 0, st, a);
+    else if (f32 && e.aos)
       hipLaunchKernelGGL((heis_sweep_fast<float, ZC, true>), dim3(e.R), dim3(e.threads), e.aos_bytes, st, a);
     else if (f32) hipLaunchKernelGGL((heis_sweep_fast<float, ZC, false>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((heis_sweep_fast<double, ZC, false>), dim3(e.R), dim3(e.threads), sh, st, a);
@@ -1480,6 +1575,16 @@ static void set_lds_attr(MCEngine &e) {
 }
 
 static void launch_sweep(MCEngine &e, hipStream_t st) {
+  const SweepArgs &a = e.args;
+  if (e.stat && e.fast && !e.compact && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0 &&
+      e.lat.Z > 16 && e.lat.Z <= 18) {
+    // eighteen coupled slots (triangular lattice with three shells): the
+    // statically placed kernel with exactly nine label words per site
+    e.args.config_base = 0;
+    hipLaunchKernelGGL((heis_sweep_fast<float, 18, true, false, true>), dim3(e.R), dim3(e.threads), 0,
+                       st, e.args);
+    return;
+  }
   HSMD_ZC_DISPATCH(e.lat.Z, launch_sweep_zc, e, st);
 }
 

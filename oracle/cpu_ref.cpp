@@ -571,6 +571,93 @@ void orc_mc_replay(void *mp, int model, int R, const double *temps, const int32_
   }
 }
 
+}  // extern "C"
+
+namespace {
+
+// --- heat-bath replay of the batched FP32 sweep ----------------------------
+// The FP32 kernel's heat bath evaluated in FP64, for a check of that kernel's
+// indexing and formulas at full size (the reference's own formula is
+// heat_bath() above; both sample the same distribution).  Uniforms: 24-bit,
+// u = (w >> 8) 2^-24 of Philox words; t = 1 + cos(theta) =
+// -log1p(u2 expm1(-2 b H)) / (b H) clamped to [0, 2]; the orthonormal frame
+// about h of Duff et al. (JCGT 2017).
+void heat_bath_f32form(double beta, const double h[3], double u1, double u2, double o[3]) {
+  const double H2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+  double t, ux = 0, uy = 0, uz = 1;
+  bool frame = false;
+  if (H2 > 1.0e-24) {
+    const double H = sqrt(H2), bh = beta * H;
+    t = -log1p(u2 * expm1(-2.0 * bh)) / bh;
+    ux = h[0] / H; uy = h[1] / H; uz = h[2] / H;
+    frame = true;
+  } else {
+    t = 2.0 * u2;
+  }
+  t = std::min(std::max(t, 0.0), 2.0);
+  const double chi = t - 1.0, w = sqrt(t * (2.0 - t));
+  const double s0 = w * cos(2.0 * PI * u1), s1 = w * sin(2.0 * PI * u1);
+  if (frame) {
+    const double sg = uz >= 0.0 ? 1.0 : -1.0;
+    const double a = -1.0 / (sg + uz), b = ux * uy * a;
+    const double e1[3] = {1.0 + sg * ux * ux * a, sg * b, -sg * ux};
+    const double e2[3] = {b, sg + uy * uy * a, -uy};
+    for (int d = 0; d < 3; d++) o[d] = s0 * e1[d] + s1 * e2[d] + chi * (d == 0 ? ux : d == 1 ? uy : uz);
+  } else {
+    o[0] = s0; o[1] = s1; o[2] = chi;
+  }
+}
+
+double u24(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0); }
+
+}  // namespace
+
+extern "C" {
+// n_hb heat-bath passes of sweep `sweep` (no overrelaxation, no exchange) of
+// R configurations (configuration c at temperature temps[c]) in the FP32
+// kernel's order: colour classes c = 0 .. nc - 1 in turn, thread t of class
+// c updating site order[col_off[c] + t].  Philox counter (site, sweep * 16 +
+// pass, c, 0): a thread draws a block for its site of an even class and uses
+// words x, y; words z, w serve its site of the next (odd) class; a site of an
+// odd class whose thread had no site in the class before draws its own block.
+void orc_mc_replay_hb32(void *mp, int R, const double *temps, const int32_t *order,
+                        const int32_t *col_off, int nc, int n_hb, uint64_t seed, uint32_t sweep,
+                        double *spins) {
+  Model &m = *(Model *)mp;
+  const int N = m.N;
+  for (int c = 0; c < R; c++) {
+    double *S = spins + (size_t)c * N * 3;
+    const double b = 1.0 / temps[c];
+    for (int pass = 0; pass < n_hb; pass++) {
+      const uint32_t step = sweep * 16u + (uint32_t)pass;
+      std::vector<uint32_t> keep_z, keep_w;
+      std::vector<char> keep;
+      for (int col = 0; col < nc; col++) {
+        const int n = col_off[col + 1] - col_off[col];
+        std::vector<uint32_t> nz(n), nw(n);
+        std::vector<char> nk(n, 0);
+        for (int t = 0; t < n; t++) {
+          const int i = order[col_off[col] + t];
+          double h[3], o[3];
+          local_field(m, S, i, h);
+          uint32_t r0, r1;
+          if ((col & 1) && t < (int)keep.size() && keep[t]) {
+            r0 = keep_z[t]; r1 = keep_w[t];
+          } else {
+            const W4 r = philox((uint32_t)i, step, (uint32_t)c, 0u, seed);
+            r0 = r.v[0]; r1 = r.v[1];
+            if (!(col & 1)) { nz[t] = r.v[2]; nw[t] = r.v[3]; nk[t] = 1; }
+          }
+          heat_bath_f32form(b, h, u24(r0), u24(r1), o);
+          S[3 * i] = o[0]; S[3 * i + 1] = o[1]; S[3 * i + 2] = o[2];
+        }
+        if (!(col & 1)) { keep_z = nz; keep_w = nw; keep = nk; }
+        else keep.clear();
+      }
+    }
+  }
+}
+
 // --- reference-order MC (statistical oracle and CPU baseline) --------------
 // Per sweep and replica: n_or overrelaxation passes then n_hb heat-bath
 // passes (Heisenberg) or one Metropolis pass (Ising), each over a fresh

@@ -39,6 +39,8 @@ _SIG = {
                             _I32]),
     "orc_mc_replay": (None, [_P, _I, _I, _D, _I32, _I, _I, _I, _I, _I, ctypes.c_uint64,
                              ctypes.c_uint32, _I, _I, _D, _I32, _I64, _I64, _I32, _D]),
+    "orc_mc_replay_hb32": (None, [_P, _I, _D, _I32, _I32, _I, _I, ctypes.c_uint64, ctypes.c_uint32,
+                                  _D]),
     "orc_mc_reference": (None, [_P, _I, _I, _D, _I, _I, _I, _I, _I, ctypes.c_uint64, _I, _I,
                                 _D, _I32, _D, _I, _I64, _I64, _I32, _I]),
     "orc_smd_run": (None, [_P, _D, ctypes.c_double, _I, _I, _D, _D, _D]),
@@ -166,6 +168,16 @@ class Model:
                             cnt.ctypes.data_as(_I64), exc.ctypes.data_as(_I64), _ip(boost),
                             _dp(tr))
         return tr, state
+
+    def mc_replay_hb32(self, temps, order, col_off, spins, n_hb=1, seed=1, sweep=0):
+        """Heat-bath passes of the batched FP32 sweep replayed in FP64 (the
+        FP32 kernel's formulas and uniforms); spins (R, N, 3), configuration
+        c at temps[c], updated in place."""
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        od = np.ascontiguousarray(order, dtype=np.int32)
+        co = np.ascontiguousarray(col_off, dtype=np.int32)
+        lib().orc_mc_replay_hb32(self._h, len(temps), _dp(temps), _ip(od), _ip(co), len(co) - 1,
+                                 int(n_hb), int(seed), int(sweep) & 0xffffffff, _dp(spins))
 
     def mc_reference(self, model, temps, spins, n_sweeps, n_or=5, n_hb=5, n_or_b=2, n_hb_b=8,
                      cap50=True, seed=1, pt_start=100, threads=0, state=None, sweep0=0):

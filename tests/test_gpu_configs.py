@@ -375,6 +375,38 @@ def test_config3_fp32_overrelaxation_sweep_matches_fp64_replay():
     assert err < 1e-3
 
 
+def test_config3_fp32_heat_bath_sweep_matches_its_fp64_restatement():
+    """One heat-bath pass (no overrelaxation) of all 256 FP32 replicas at
+    full size against the oracle's FP64 evaluation of the FP32 kernel's own
+    heat bath (its uniforms, the Philox block shared by a thread's sites of
+    an even and the next odd colour class, its inverse CDF and frame): pins
+    the kernel's random-stream indexing and heat-bath arithmetic at
+    configuration 3.  The heat bath contracts differences, so FP32 rounding
+    stays near 1e-6."""
+    _say("config3 fp32 HB replay")
+    c, lat, temps = _config3()
+    m = _model(lat, c)
+    mc = H.MonteCarlo(lat, temps, precision=H.BATCHED_F32, n_or=0, n_hb=1, n_or_boost=0,
+                      n_hb_boost=1, seed=3)
+    rng = np.random.default_rng(309)
+    s = _unit(rng, (256, lat.N, 3))
+    mc.set_spins(s)
+    mc.run(1, pt_parity=-1)
+    col = lat.colours()
+    order = np.argsort(col, kind="stable").astype(np.int32)   # uniform degree: site order per class
+    col_off = np.concatenate([[0], np.cumsum(np.bincount(col))]).astype(np.int32)
+    ref = s.copy()
+    m.mc_replay_hb32(temps, order, col_off, ref, n_hb=1, seed=3, sweep=0)
+    d = np.abs(mc.get_spins() - ref).max(axis=-1)
+    err, q = d.max(), np.quantile(d, 0.9999)
+    far = (d > 1e-3).mean()
+    _say("max |dS| %.2e, 99.99 %% quantile %.2e, fraction above 1e-3 %.1e" % (err, q, far))
+    # the frame about h changes sign with h_z: where h_z is within rounding of
+    # 0 the two precisions may pick opposite frames (one site in ~1e6)
+    assert q < 1e-4
+    assert far < 1e-5
+
+
 @pytest.mark.parametrize("prec", [H.BATCHED_F64, H.PLAIN_F64])
 def test_config3_fp64_replay_of_four_replicas(prec):
     """Configuration 3's lattice and couplings in FP64 (a replica of

@@ -54,6 +54,10 @@ HBM_PEAK_GBS = 8000.0
 # time points per DSSF trajectory chunk of the batched FP32 SMD engine: the
 # contraction runs when a chunk fills, so timed windows hold whole chunks
 SMD_CHUNK = 2048
+# DSSF time-contraction kernel per contraction tile
+GEMM_KERNEL = {0: "time_gemm_mfma_q_kernel", 1: "time_gemm_mfma_kernel",
+               2: "time_gemm_mfma_q256_kernel", 3: "time_gemm_mfma_db_kernel",
+               4: "time_gemm_mfma_w_kernel"}
 # RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
 SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
                    3: "smd_kernel_wide<float,12,false>"}
@@ -627,6 +631,7 @@ def bench_smd(timer, args, rng, tune):
     # FP32 trajectory row the contraction reads back (3 N values per sample)
     rk_hbm = sum(4 * 3 * lats[i].N for i in idx)
     form = smd.tuning["form"] if smd.tuning else 2
+    gemm = smd.tuning["gemm"] if smd.tuning else 4
     roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
                          SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % (SMD_CHUNK - 1),
                          per=1.0 / (SMD_CHUNK - 1),
@@ -647,7 +652,7 @@ def bench_smd(timer, args, rng, tune):
                                             "peak": 2500.0, "unit": "TFLOP/s",
                                             "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
                                             "traffic": read_traffic("dssf_contraction_mfma_pmc.txt"),
-                                            "kernel": "time_gemm_mfma_q_kernel",
+                                            "kernel": GEMM_KERNEL.get(gemm, "?"),
                                             "algorithmic_flop_per_launch": alg_flop,
                                             "executed_flop_per_launch": mfma_flop,
                                             "executed_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,

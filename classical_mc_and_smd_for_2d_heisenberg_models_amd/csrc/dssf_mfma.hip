@@ -5,18 +5,23 @@
 // where A is a chunk of the trajectory (time k, spin component x site m,
 // FP32) and B the windowed DFT twiddles (time k, frequency bin j with cos in
 // the first n_omega columns and -sin in the rest).  Each FP32 operand is split
-// into two FP16 halves, x = x_hi + x_lo with x_hi = fp16(x) and
-// x_lo = fp16(x - x_hi), and the product is formed as
+// into two FP16 halves, x = x_hi + x_lo with x_lo = fp16(x - x_hi), and the
+// product is formed as
 //   hi*hi + hi*lo + lo*hi     (the lo*lo term is below FP32 rounding)
-// with three v_mfma_f32_16x16x16_f16 per 16x16x16 block and FP32 accumulation,
-// which keeps about 21 significant bits -- FP32-level accuracy for operands in
-// [-1, 1].  This replaces the reference's per-q MKL FFT over time
-// (CF/subrouts.cpp:12062-12105); the DFT is exact for every bin kept.
+// with three MFMAs per block and FP32 accumulation, which keeps about 21
+// significant bits -- FP32-level accuracy for operands in [-1, 1].  (Emulated
+// on the CPU for a 2381-point trajectory of the square L=48 / kagome L=32
+// batch, on the bins above 1e-2 of their peak: one FP16 product leaves a
+// relative error of 3.7e-3, hi*hi + lo*hi with FP16 twiddles 4e-4, the three
+// products 8e-7.)  This
+// replaces the reference's per-q MKL FFT over time (CF/subrouts.cpp:12062-12105);
+// the DFT is exact for every bin kept.
 //
-// Two kernels: time_gemm_mfma_kernel, a 64 x 64 tile with on-the-fly
-// twiddle splitting (v_mfma_f32_16x16x16_f16), kept for bin counts that are
-// not a multiple of 64; and time_gemm_mfma_q_kernel, the main one, described
-// where it is defined.
+// Kernels: time_gemm_mfma_kernel, a 64 x 64 tile with on-the-fly splitting
+// (x_hi = fp16(x), v_mfma_f32_16x16x16_f16), kept for bin counts that are not
+// a multiple of 128; the 128-row tiles (time_gemm_mfma_q / q256 / db /
+// w kernels: x_hi = x cut to 11 significant bits, v_mfma_f32_16x16x32_f16,
+// twiddles pre-split on the host), each described where it is defined.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -35,6 +40,16 @@ struct GemmSample {
 };
 
 constexpr int TM = 64, TJ = 64, TK = 16, KP = TK + 4;  // KP: padded k stride (halves)
+
+// Y update of one accumulator: the first chunk of a run (k0 == 0) stores, later
+// chunks add with a no-return global atomic.  Each element of Y has exactly one
+// owning lane per launch, so the sum is the same as a read-add-write, but the
+// wave issues its 128 updates without waiting for reads of Y (a read-modify-
+// write epilogue left the matrix cores idle for tens of microseconds per tile).
+__device__ __forceinline__ void y_update(float *p, float v, bool first) {
+  if (first) *p = v;
+  else unsafeAtomicAdd(p, v);
+}
 
 __global__ __launch_bounds__(256) void time_gemm_mfma_kernel(const GemmSample *samples,
                                                              const float *__restrict__ A,
@@ -107,7 +122,7 @@ __global__ __launch_bounds__(256) void time_gemm_mfma_kernel(const GemmSample *s
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
-        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+        if (m < M) y_update(&Yb[(size_t)m * ldb + j], acc[a][b][r], k0 == 0);
       }
     }
 }
@@ -271,7 +286,7 @@ __device__ __forceinline__ void time_gemm_q_body(const GemmSample *samples, cons
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
-        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+        if (m < M) y_update(&Yb[(size_t)m * ldb + j], acc[a][b][r], k0 == 0);
       }
     }
 }
@@ -408,9 +423,153 @@ void time_gemm_mfma_db_kernel(const GemmSample *samples, const float *__restrict
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
-        if (m < M) Yb[(size_t)m * ldb + j] += acc[a][b][r];
+        if (m < M) y_update(&Yb[(size_t)m * ldb + j], acc[a][b][r], k0 == 0);
       }
     }
+}
+
+// Wide variant (2 n_omega a multiple of 512): a workgroup of 8 waves owns a
+// 128 (m) x 512 (j) tile of Y, each wave a 64 x 128 sub-tile of 4 x 8 MFMA
+// blocks (128 accumulator registers), so at 512 bins the trajectory is read
+// twice per chunk instead of eight times, and each trajectory value is split
+// into its FP16 halves twice instead of eight times.  The twiddles come in a
+// chunk-major layout, [t / 8][2 n_omega][8] FP16 hi and lo: the eight time
+// points of one column are 16 bytes, and a wave's 64 lanes take 64
+// consecutive columns of one time chunk -- 1 KB contiguous in memory and in
+// LDS -- so the loads are coalesced, the LDS writes conflict-free, and the
+// MFMA B fragments (16 columns x 8 time points per 16-lane group) read 256
+// contiguous bytes per lane group.  Step k + 1's trajectory tile (128 rows x
+// 32 time points, FP32) and twiddles are loaded into registers while step
+// k's MFMAs run, then split / stored to the single LDS stage (20 KB +
+// 64 KB).  One workgroup per CU, 2 waves per SIMD.
+constexpr int W_TJ = 512, W_THREADS = 512;
+constexpr int W_BSTAGE = 4 * W_TJ * 8;   // halves per (hi or lo) twiddle stage: 4 chunks x 512 columns x 8
+
+__global__ __launch_bounds__(W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void time_gemm_mfma_w_kernel(const GemmSample *samples, const float *__restrict__ A,
+                             const _Float16 *__restrict__ B8h, const _Float16 *__restrict__ B8l,
+                             int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 Ah[QM * WKP], Al[QM * WKP];
+  __shared__ __align__(16) _Float16 Bs[2][W_BSTAGE];   // [hi, lo][chunk][col][8]
+  const GemmSample sd = samples[blockIdx.z];
+  const int M = sd.M, lda = sd.lda;
+  const int nj = ldb / W_TJ;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * W_TJ;
+  if (m0 >= M) return;
+  const float *Ab = A + sd.a_off;
+  float *Yb = Y + sd.y_off;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = (wave >> 2) * 64, wj = (wave & 3) * 128;
+  const int li = lane & 15, lg = lane >> 4;
+  // trajectory staging: thread t owns row am = t % 128, time points 8 kh .. 8 kh + 7
+  const int am = tid % QM;
+  const int kh = __builtin_amdgcn_readfirstlane(tid / QM);   // uniform per wave
+  const bool arow = m0 + am < M;
+  const __amdgpu_buffer_rsrc_t arsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)Ab, (short)0, 0x7fffffff, 0x00020000);
+  const int acol = 4 * min(m0 + am, lda - 1);
+  const int lda4 = 4 * lda;
+  // twiddle staging: wave w copies region w of the stage (time chunk w % 4 of
+  // the hi table for w < 4, of the lo table otherwise), eight 1 KB pieces
+  const int bchunk = wave & 3;
+  const uint4 *bsrc = (const uint4 *)((wave < 4 ? B8h : B8l) + (size_t)bchunk * ldb * 8) + j0 + lane;
+  uint4 *bdst = (uint4 *)&Bs[wave >> 2][bchunk * W_TJ * 8] + lane;
+  float4_t acc[4][8];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  float ra[8];
+  // named registers, not an array: an array captured by the lambdas below
+  // would be placed in LDS by the compiler, 128 bytes per thread
+  uint4 rb0, rb1, rb2, rb3, rb4, rb5, rb6, rb7;
+  auto fetch = [&](int kk) {
+    if (kk + WK <= K) {
+      const int r0 = (kk + 8 * kh) * lda4;
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        ra[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(arsrc, acol, r0 + e * lda4, 0));
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int ka = kk + 8 * kh + e;
+        ra[e] = (arow && ka < K) ? Ab[(size_t)ka * lda + m0 + am] : 0.f;
+      }
+    }
+    // time chunk (k0 + kk) / 8 + bchunk, columns j0 .. j0 + 511
+    const uint4 *g = bsrc + (size_t)((k0 + kk) / 8) * ldb;
+    rb0 = g[0]; rb1 = g[64]; rb2 = g[128]; rb3 = g[192];
+    rb4 = g[256]; rb5 = g[320]; rb6 = g[384]; rb7 = g[448];
+  };
+  auto stage = [&]() {
+    half8_t hv, lv;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      // hi: the value cut to FP16's 11 significant bits (exact in FP16 for
+      // |x| >= 2^-14); lo: the exact FP32 remainder, rounded to FP16
+      const float x = ra[e];
+      const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) & 0xffffe000u);
+      hv[e] = (_Float16)h;
+      lv[e] = (_Float16)(x - h);
+    }
+    *(half8_t *)&Ah[am * WKP + 8 * kh] = hv;
+    *(half8_t *)&Al[am * WKP + 8 * kh] = lv;
+    bdst[0] = rb0; bdst[64] = rb1; bdst[128] = rb2; bdst[192] = rb3;
+    bdst[256] = rb4; bdst[320] = rb5; bdst[384] = rb6; bdst[448] = rb7;
+  };
+  fetch(0);
+  for (int kk = 0; kk < K; kk += WK) {
+    stage();
+    __syncthreads();
+    if (kk + WK < K) fetch(kk + WK);
+    half8_t ah[4], al[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int row = wm + a * 16 + li;
+      ah[a] = *(const half8_t *)&Ah[row * WKP + 8 * lg];
+      al[a] = *(const half8_t *)&Al[row * WKP + 8 * lg];
+    }
+    const _Float16 *bh0 = &Bs[0][lg * W_TJ * 8];
+    const _Float16 *bl0 = &Bs[1][lg * W_TJ * 8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int col = wj + b * 16 + li;
+      const half8_t bh = *(const half8_t *)&bh0[col * 8];
+      const half8_t bl = *(const half8_t *)&bl0[col * 8];
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh, acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int j = j0 + wj + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + a * 16 + lg * 4 + r;
+        if (m < M) y_update(&Yb[(size_t)m * ldb + j], acc[a][b][r], k0 == 0);
+      }
+    }
+}
+
+// B8h / B8l: twiddles in the chunk-major layout [T' / 8][ldb][8] (T' a multiple of 32)
+void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const float *A,
+                             const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
+                             float *Y, hipStream_t st) {
+  if (ldb % W_TJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 512");
+  if (k0 % WK != 0) throw Error(HSMD_ERR_ARG, "trajectory chunks start at multiples of 32 points");
+  const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
+  dim3 grid(nm * (ldb / W_TJ), 1, S);
+  hipLaunchKernelGGL(time_gemm_mfma_w_kernel, grid, dim3(W_THREADS), 0, st,
+                     (const GemmSample *)samples_dev, A, B8h, B8l, k0, K, ldb, Y);
 }
 
 // tj: 128 or 256 columns per workgroup tile; db: the double-buffered 128 tile

@@ -158,6 +158,9 @@ __global__ void dft_cols_kernel(int n_batch, int G, int n_omega, const T *R, con
 void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float *A, const float *B,
                            int ldb, int k0, int K, float *Y, hipStream_t st);
 
+void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const float *A,
+                             const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
+                             float *Y, hipStream_t st);
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
                                 int ldb, float *Y, hipStream_t st, int tj, bool db);
@@ -206,6 +209,7 @@ struct SMDEngine {
   DevBuf<int> sample_ids;
   DevBuf<GemmSampleHost> gemm_samples;
   DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 nop][ldt]
+  DevBuf<_Float16> tw8_hi, tw8_lo; // the same, chunk-major [ldt / 8][2 nop][8] (2 nop % 512 == 0)
   int nop = 0;                     // frequency bins held internally (n_omega padded)
   int ldt = 0;
   bool set = false;
@@ -215,7 +219,8 @@ struct SMDEngine {
   std::vector<double> feed0;
   // launch shape (hsmd_smd_set_shape): LDS-resident form possible, one slot
   // class for every workgroup, and the contraction tile (0: 128 x 128 with
-  // pre-split twiddles, 1: 64 x 64)
+  // pre-split twiddles, 1: 64 x 64, 2: 128 x 256, 3: 128 x 128 with two
+  // LDS stages, 4: 128 x 512 with chunk-major twiddles)
   bool lds_ok = false;
   bool one_class = false;
   int gemm = 0;
@@ -252,6 +257,19 @@ static void upload_twiddles(SMDEngine &e) {
       }
     e.tw_hi.upload(hi.data(), hi.size());
     e.tw_lo.upload(lo.data(), lo.size());
+    if ((2 * no) % 512 == 0) {
+      // chunk-major copy: the eight time points of one column are 16 bytes,
+      // consecutive columns of one 8-point chunk are consecutive
+      std::vector<_Float16> h8(hi.size()), l8(lo.size());
+      for (int j = 0; j < 2 * no; j++)
+        for (int t = 0; t < e.ldt; t++) {
+          const size_t dst = ((size_t)(t / 8) * 2 * no + j) * 8 + (t % 8);
+          h8[dst] = hi[(size_t)j * e.ldt + t];
+          l8[dst] = lo[(size_t)j * e.ldt + t];
+        }
+      e.tw8_hi.upload(h8.data(), h8.size());
+      e.tw8_lo.upload(l8.data(), l8.size());
+    }
   }
 }
 
@@ -516,7 +534,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
                e->spt * (int)es <= 16;
   e->lds_ok = e->use_lds;
-  e->gemm = (e->f32 && e->ldt > 0) ? 0 : 1;
+  e->gemm = e->tw8_hi.n > 0 ? 4 : (e->f32 && e->ldt > 0) ? 0 : 1;
   // register-resident global form only for the plain form's small lattices
   e->wide = !e->use_lds && (sp.precision != HSMD_PLAIN_F64 || e->spt > (e->f32 ? 8 : 4));
   if (!e->use_lds) {
@@ -685,7 +703,10 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.f32 && e.ldt > 0 && e.gemm != 1)
+  if (e.gemm == 4)
+    launch_time_gemm_mfma_w(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw8_hi.p,
+                            e.tw8_lo.p, e.chunk_base, e.chunk_fill, ldb, (float *)e.ybuf.p, 0);
+  else if (e.f32 && e.ldt > 0 && e.gemm != 1)
     launch_time_gemm_mfma_wide(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw_hi.p,
                                e.tw_lo.p, e.ldt, e.chunk_base, e.chunk_fill, ldb,
                                (float *)e.ybuf.p, 0, e.gemm == 2 ? 256 : 128, e.gemm == 3);
@@ -970,18 +991,21 @@ static int current_form(const SMDEngine &e) {
 // batches mixing two classes); 3 = global-memory form (one workgroup per
 // sample, stages in HBM / Infinity Cache).  gemm: 0 = 128 x 128 contraction
 // tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64,
-// 2 = 128 x 256 with pre-split twiddles, 3 = 128 x 128 with two LDS stages.
+// 2 = 128 x 256 with pre-split twiddles, 3 = 128 x 128 with two LDS stages,
+// 4 = 128 x 512 with chunk-major twiddles (2 n_omega a multiple of 512).
 // Per-site arithmetic is the same in every shape; FP64 reductions may group
 // terms differently.
 void smd_set_shape(SMDEngine &e, int form, int gemm) {
   if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
   if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
-  if (gemm < 0 || gemm > 3) throw Error(HSMD_ERR_ARG, "gemm tile must be 0, 1, 2 or 3");
+  if (gemm < 0 || gemm > 4) throw Error(HSMD_ERR_ARG, "gemm tile must be 0..4");
   if ((form == 1 || form == 2) && !e.lds_ok)
     throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
   if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
   if (gemm != 1 && !(e.f32 && e.ldt > 0))
     throw Error(HSMD_ERR_ARG, "the 128-row MFMA tiles need FP32 and 2 n_omega a multiple of 256");
+  if (gemm == 4 && e.tw8_hi.n == 0)
+    throw Error(HSMD_ERR_ARG, "the 128 x 512 MFMA tile needs FP32 and 2 n_omega a multiple of 512");
   if (form == 1 || form == 2) {
     e.use_lds = true;
     e.wide = false;
@@ -1083,6 +1107,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
     if (sp.dssf && e->f32) {
       std::vector<int> tiles = {1};
       if (e->ldt > 0) tiles = {0, 3, 2, 1};
+      if (e->tw8_hi.n > 0) tiles.insert(tiles.begin(), 4);
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);

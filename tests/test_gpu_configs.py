@@ -514,7 +514,7 @@ def _feed_blocks(smd, trajs, step=256):
         smd.feed(np.concatenate([tr[t0:t1] for tr in trajs], axis=1))
 
 
-@pytest.mark.parametrize("gemm", [0, 2, 3, 1])
+@pytest.mark.parametrize("gemm", [4, 0, 2, 3, 1])
 def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm):
     """The split-FP16 MFMA contraction's own error: the oracle's FP64
     trajectories (square L=48 with its 230 vacancies and kagome L=32, two
@@ -563,7 +563,7 @@ def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm):
                 assert err < 1e-9
 
 
-@pytest.mark.parametrize("gemm", [0, 2, 3, 1])
+@pytest.mark.parametrize("gemm", [4, 0, 2, 3, 1])
 def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm):
     """The same batch integrated by both engines: FP32 RK4 with the MFMA
     contraction against FP64 RK4 with the FP64 GEMM over 2381 time points

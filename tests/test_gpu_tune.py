@@ -109,7 +109,7 @@ def test_smd_tuner_and_shapes_keep_parity(tmp_path, monkeypatch):
     sd = H.SpinDynamics(lats, idx, dt, T, n_omega=128, tune=True, tune_steps=16)
     t = sd.tuning
     assert t["cached"] == 0 and t["n_candidates"] >= 3
-    assert t["form"] in (1, 2, 3) and t["gemm"] in (0, 1, 2, 3)
+    assert t["form"] in (1, 2, 3) and t["gemm"] in (0, 1, 2, 3)   # 2 n_omega = 256: no 512 tile
     assert t["ms_per_step"] <= t["default_ms_per_step"] and t["gemm_ms"] <= t["default_gemm_ms"]
     sd2 = H.SpinDynamics(lats, idx, dt, T, n_omega=128, tune=True, tune_steps=16)
     assert sd2.tuning["cached"] == 1
@@ -134,6 +134,10 @@ def test_shape_errors():
     e = H.SpinDynamics([lat], np.zeros(2, np.int32), 0.01, 64, n_omega=64)
     with pytest.raises(H.HSMDError):
         e.set_shape(2, 1)      # one slot class only
+    with pytest.raises(H.HSMDError):
+        e.set_shape(0, 4)      # 2 n_omega = 128: no 512-column tile
+    with pytest.raises(H.HSMDError):
+        e.set_shape(0, 5)
     e64 = H.SpinDynamics([lat], np.zeros(2, np.int32), 0.01, 64, n_omega=64, precision=H.BATCHED_F64)
     with pytest.raises(H.HSMDError):
         e64.set_shape(1, 0)    # the 128 x 128 tile is FP32 only

@@ -51,9 +51,10 @@ sys.path.insert(0, ROOT)
 import classical_mc_and_smd_for_2d_heisenberg_models_amd as H  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# time points per DSSF trajectory chunk of the batched FP32 SMD engine: the
-# contraction runs when a chunk fills, so timed windows hold whole chunks
-SMD_CHUNK = 2048
+# the batched FP32 SMD engine's longest DSSF trajectory chunk: the
+# contraction runs when a chunk fills, and the configuration-4 and -5 runs
+# (8192 and 4096 points) each fit in one chunk; timed runs cover whole runs
+SMD_CHUNK = 8192
 # DSSF time-contraction kernel per contraction tile
 GEMM_KERNEL = {0: "time_gemm_mfma_q_kernel", 1: "time_gemm_mfma_kernel",
                2: "time_gemm_mfma_q256_kernel", 3: "time_gemm_mfma_db_kernel",
@@ -201,7 +202,7 @@ def q_path(n=200, qmax=2 * math.pi):
     return out
 
 
-def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=SMD_CHUNK):
+def bench_graph(timer, rng, tune, mc_steps=50):
     c, lat, box = graph_workload(rng)
     N = lat.N
     print("[bench] graph: N=%d box=%.1f Z=%d colours=%d" % (N, box, lat.zn1 + lat.zn2 + lat.znx,
@@ -221,15 +222,16 @@ def bench_graph(timer, rng, tune, mc_steps=50, smd_steps=SMD_CHUNK):
     smd = H.SpinDynamics([lat], np.zeros(S, np.int32), 0.01, T, n_omega=512,
                          precision=H.BATCHED_F32, tune=tune, tune_steps=64)
     stune = smd.tuning
-    smd.set_samples([unit_spins(rng, (N, 3)) for _ in range(S)])
-    smd.step(SMD_CHUNK - 1)
+    init = [unit_spins(rng, (N, 3)) for _ in range(S)]
+    smd.set_samples(init)
+    smd.step(16)        # warm-up
+    # the whole 4096-point run, its DSSF contraction included
+    smd_steps = T - 1
+    smd.set_samples(init)
     timer.start()
     smd.step(smd_steps)
     sms = timer.stop()
     srate = S * N * smd_steps / (sms * 1e-3)
-    rest = T - 1 - (SMD_CHUNK - 1) - smd_steps
-    if rest > 0:
-        smd.step(rest)
     q = q_path(200)
     timer.start()
     d = smd.dssf_q(q)
@@ -581,49 +583,40 @@ def bench_config1(timer, rng):
     return out
 
 
-def smd_time_points(k):
-    """Time points of the configuration-4 run that times k steps after one
-    warm-up chunk, completes the chunk, then times one more chunk of RK4
-    steps and two single steps."""
-    fill = (-k) % SMD_CHUNK
-    return SMD_CHUNK + k + fill + (SMD_CHUNK - 1) + 2 + 1
-
-
 def bench_smd(timer, args, rng, tune):
     lats, idx, samples = smd_workload(rng)
     sites = sum(lats[i].N for i in idx)
-    k = args.smd_steps
-    T = max(8192, smd_time_points(k))
+    k = args.smd_steps           # RK4 steps of the run: n_steps = k + 1 time points
+    T = k + 1
+    chunk = min(T, SMD_CHUNK)    # FP32 trajectory chunk (memory permitting)
     smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
                          tune=tune, tune_steps=64)
+    # breakdown run (also the warm-up): the RK4 kernel alone over 1000 steps
+    # (one launch, no chunk fills), then the step that fills the chunk (one
+    # RK4 step + the contraction of the whole chunk) against a plain step
     smd.set_samples(samples)
-    smd.step(SMD_CHUNK - 1)  # warm-up: time points 0..2047, one full trajectory chunk
+    nb = min(1000, chunk - 3)
     timer.start()
-    smd.step(k)     # k RK4 steps; every SMD_CHUNK points the DSSF chunk GEMM runs
-    sms = timer.stop()
-    srate = sites * k / (sms * 1e-3)
-    # complete the current trajectory chunk (its flush runs the GEMM)
-    fill = k % SMD_CHUNK
-    if fill:
-        smd.step(SMD_CHUNK - fill)
-    # the RK4 kernel alone: one launch advances every sample 2047 steps
-    # inside the next chunk (no flush)
-    timer.start()
-    smd.step(SMD_CHUNK - 1)
-    rk_ms = timer.stop() / (SMD_CHUNK - 1)
-    # the chunk GEMM alone: the step that fills the chunk (one RK4 step +
-    # the flush) against a plain step
+    smd.step(nb)
+    rk_ms = timer.stop() / nb
+    smd.step(chunk - 2 - nb)
     timer.start()
     smd.step(1)
     t_flush = timer.stop()
+    gemm_ms = max(t_flush - rk_ms, 1e-6)
+    if T > chunk:
+        smd.step(T - 1 - (chunk - 1))
+    # the timed run: all k steps of the configuration from the initial samples,
+    # the DSSF contraction of every chunk included
+    smd.set_samples(samples)
     timer.start()
-    smd.step(1)
-    t_one = timer.stop()
-    gemm_ms = max(t_flush - t_one, 1e-6)
+    smd.step(k)
+    sms = timer.stop()
+    srate = sites * k / (sms * 1e-3)
     gm = sum(3 * lats[i].N for i in idx)
-    mfma_flop = 2.0 * gm * 1024 * SMD_CHUNK * 3      # M x (2 n_omega) x K, three split products
+    mfma_flop = 2.0 * gm * 1024 * chunk * 3      # M x (2 n_omega) x K, three split products
     # algorithmic DSSF contraction FLOP per chunk: 8 x 3 x Nq x n_omega x T per sample
-    alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * SMD_CHUNK for i in idx)
+    alg_flop = sum(8.0 * 3 * lats[i].Nq * 512 * chunk for i in idx)
     # gather model per site-RK4-step: 6 field passes x [(z_i + 2) 12 B + 4 z_i B],
     # summed over sites: 6 (24 N + 32 n_bonds) per sample
     rk_gather = sum(6 * (24 * lats[i].N + 32 * lats[i].n_bonds) for i in idx)
@@ -633,8 +626,8 @@ def bench_smd(timer, args, rng, tune):
     form = smd.tuning["form"] if smd.tuning else 2
     gemm = smd.tuning["gemm"] if smd.tuning else 4
     roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
-                         SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % (SMD_CHUNK - 1),
-                         per=1.0 / (SMD_CHUNK - 1),
+                         SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % nb,
+                         per=1.0 / nb,
                          extra={"per": "RK4 step of the 512-sample batch",
                                 "gather_model": {"bytes_per_step": rk_gather,
                                                  "achieved": rk_gather / (rk_ms * 1e-3) / 1e9,
@@ -646,7 +639,7 @@ def bench_smd(timer, args, rng, tune):
                        "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
                        "correction, DSSF time contraction over 512 bins included",
            "tuning": smd.tuning, "roofline": roof,
-           "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": SMD_CHUNK,
+           "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": chunk,
                                "roofline": {"bound": "mfma",
                                             "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
                                             "peak": 2500.0, "unit": "TFLOP/s",
@@ -742,7 +735,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--smd-steps", type=int, default=SMD_CHUNK)
+    ap.add_argument("--smd-steps", type=int, default=SMD_CHUNK - 1,
+                    help="RK4 steps of the configuration-4 run (8191: its 8192 time points)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-smd", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -759,8 +753,8 @@ def main():
               "blocks and sample sets would shard across GPUs, but no multi-GPU path "
               "is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.smd_steps < 1:
-        print("--smd-steps must be >= 1", file=sys.stderr)
+    if args.smd_steps < 1000:
+        print("--smd-steps must be >= 1000", file=sys.stderr)
         sys.exit(2)
     if H.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)

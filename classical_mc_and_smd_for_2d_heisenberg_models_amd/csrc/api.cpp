@@ -46,7 +46,7 @@ void debug_philox(uint64_t seed, uint32_t step, uint32_t inst, uint32_t stream, 
 void debug_exp(int n, const double *x, double *y);
 
 SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
-                      const hsmd_smd_spec &sp, const double *window);
+                      const hsmd_smd_spec &sp, const double *window, int max_chunk = 0);
 void smd_set_samples(SMDEngine &e, const double *spins);
 void smd_get_samples(SMDEngine &e, double *spins);
 void smd_step(SMDEngine &e, int n);

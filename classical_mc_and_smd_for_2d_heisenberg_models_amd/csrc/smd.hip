@@ -334,8 +334,9 @@ static Lattice natural_order(const Lattice &L) {
   return o;
 }
 
+// max_chunk > 0 caps the FP32 trajectory chunk (the autotuner's scratch engine)
 SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
-                      const hsmd_smd_spec &sp, const double *window) {
+                      const hsmd_smd_spec &sp, const double *window, int max_chunk) {
   if (S < 1) throw Error(HSMD_ERR_ARG, "sample count must be >= 1");
   if (!(sp.dt > 0.0)) throw Error(HSMD_ERR_ARG, "dt must be positive");
   if (sp.n_steps < 2) throw Error(HSMD_ERR_ARG, "n_steps must be >= 2");
@@ -463,15 +464,26 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->dlats.upload(hl.data(), hl.size());
   // sizes and layout
   const int T_ = sp.n_steps;
-  // trajectory chunk: each contraction reads and writes the whole of Y
-  // (at 512 bins as large as a 1024-point chunk), so long chunks revisit it
-  // rarely; 2048 FP32 points of the 512-sample batch of 1.6M sites take
-  // 40 GB (against 1024 points: 0.084 -> 0.079 ms per step of that batch).
-  // The chunk shrinks (2048 -> 1024 -> 512 -> 256 points) while it, Y and
-  // the spin buffers would not fit in 90 % of the free device memory.
+  // trajectory chunk: the first contraction of a run stores Y and every
+  // later one adds to it (a pass over all of Y, at 512 bins as large as a
+  // 1024-point chunk), and each contraction launch pays its tile epilogues
+  // once, so long chunks are cheaper: the FP32 chunk holds up to 8192 points
+  // (the 512-sample batch of 1.6M sites: 160 GB, one contraction per
+  // 8192-step run, 151 ms against 4 x 2048-point chunks taking 198 ms).  The
+  // chunk shrinks to the next lower power of two (down to 256 points) while
+  // it, Y and the spin buffers would not fit in 90 % of the free device
+  // memory; HSMD_SMD_MAX_CHUNK caps it (a power of two >= 256).
   e->nop = e->f32 ? ((sp.n_omega + 127) / 128) * 128 : ((sp.n_omega + 31) / 32) * 32;
   const int no = e->nop;
-  e->Tc = std::min(T_, e->f32 ? 2048 : 256);
+  int cap = e->f32 ? 8192 : 256;
+  if (const char *mc = getenv("HSMD_SMD_MAX_CHUNK")) {
+    const int v = atoi(mc);
+    if (v < 256 || (v & (v - 1)) != 0)
+      throw Error(HSMD_ERR_ARG, "HSMD_SMD_MAX_CHUNK must be a power of two >= 256");
+    cap = std::min(cap, v);
+  }
+  if (max_chunk > 0) cap = std::min(cap, max_chunk);
+  e->Tc = std::min(T_, cap);
   if (sp.dssf) {
     size_t freeb = 0, totb = 0;
     HIP_CHECK(hipMemGetInfo(&freeb, &totb));
@@ -484,7 +496,9 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
     }
     // spins, t = 0 spins, stages and the wide form's increments: at most 11 copies
     const auto need = [&](int tc) { return (double)es * ((double)tc * rows + ysz + 11.0 * sp3); };
-    while (e->Tc > 256 && need(e->Tc) > 0.9 * (double)freeb) e->Tc /= 2;
+    // next lower power of two: every chunk but the first starts at a multiple of 32 points
+    const auto lower = [](int tc) { int p = 256; while (2 * p < tc) p *= 2; return p; };
+    while (e->Tc > 256 && need(e->Tc) > 0.9 * (double)freeb) e->Tc = lower(e->Tc);
     if (need(e->Tc) > 0.9 * (double)freeb)
       throw Error(HSMD_ERR_DEVICE, "the DSSF buffers of this batch exceed the free device memory");
   }
@@ -1054,7 +1068,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
     out->cached = 1;
     return;
   }
-  std::unique_ptr<SMDEngine> e(smd_create(lats, nl, sample_lattice, S, sp, window));
+  std::unique_ptr<SMDEngine> e(smd_create(lats, nl, sample_lattice, S, sp, window, 2048));
   std::vector<double> spins;
   {
     uint64_t st = 0x2545f4914f6cdd1dull;
@@ -1111,11 +1125,14 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);
+        // a contraction of up to 2048 points ranks the tiles (the cost per
+        // point is flat beyond a few hundred)
+        const int kt = std::min(std::min(e->Tc, sp.n_steps), 2048);
         e->chunk_base = 0;
-        e->chunk_fill = std::min(e->Tc, sp.n_steps);
+        e->chunk_fill = kt;
         flush_chunk(*e);   // warm-up
         e->chunk_base = 0;
-        e->chunk_fill = std::min(e->Tc, sp.n_steps);
+        e->chunk_fill = kt;
         const double ms = timed([&] { flush_chunk(*e); });
         if (getenv("HSMD_TUNE_VERBOSE")) fprintf(stderr, "[hsmd-tune] smd contraction tile %d: %.3f ms\n", g, ms);
         ncand++;

@@ -291,7 +291,11 @@ typedef struct hsmd_smd_spec {
 } hsmd_smd_spec;
 
 /* S samples; sample s evolves on lattices[sample_lattice[s]]. window[n_steps]
- * is the temporal window (nullable = "nowin"). */
+ * is the temporal window (nullable = "nowin").  With dssf set, the engine
+ * buffers trajectory chunks on the device and contracts each over time when
+ * it fills: up to 8192 points (FP32) or 256 (FP64), fewer when the chunk
+ * would not fit in 90 % of the free device memory; the environment variable
+ * HSMD_SMD_MAX_CHUNK (a power of two >= 256) caps the FP32 chunk. */
 hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
                             const int32_t *sample_lattice, int S,
                             const hsmd_smd_spec *spec, const double *window,
@@ -302,7 +306,8 @@ typedef struct hsmd_smd_tuning {
   int32_t gemm;              /* for hsmd_smd_set_shape: 0..4 */
   double ms_per_step;        /* RK4 step of the whole batch, chosen form */
   double default_ms_per_step;/* same, form chosen at creation */
-  double gemm_ms;            /* one trajectory-chunk contraction, chosen tile (-1: no DSSF) */
+  double gemm_ms;            /* contraction of one trajectory chunk of up to 2048 points,
+                                chosen tile (-1: no DSSF) */
   double default_gemm_ms;
   int32_t n_candidates;      /* shapes timed; 0 when cached */
   int32_t cached;            /* 1: read from the tuning cache (HSMD_TUNE_CACHE) */

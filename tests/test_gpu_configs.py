@@ -514,18 +514,22 @@ def _feed_blocks(smd, trajs, step=256):
         smd.feed(np.concatenate([tr[t0:t1] for tr in trajs], axis=1))
 
 
-@pytest.mark.parametrize("gemm", [4, 0, 2, 3, 1])
-def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm):
+@pytest.mark.parametrize("gemm,max_chunk", [(4, 2048), (4, None), (0, 2048), (2, 2048),
+                                            (3, 2048), (1, 2048)])
+def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm, max_chunk, monkeypatch):
     """The split-FP16 MFMA contraction's own error: the oracle's FP64
     trajectories (square L=48 with its 230 vacancies and kagome L=32, two
-    samples each, 2381 time points: one full 2048-point trajectory chunk and
-    a 333-point tail that is not a multiple of the contraction's 32-point
-    step) fed to the FP32 engine with each contraction tile, 512 bins,
-    against the FP64 DSSF of the same trajectories (FFT form of the
-    oracle's transform).  Bars: 1e-4 of the peak, and 1e-3 relative on
-    every bin above 1e-2 of its (q, component) peak; the FP64 engine fed
-    the same points stays within 1e-9."""
-    _say("config4 mfma dssf of a fed trajectory, gemm", gemm)
+    samples each, 2381 time points: with the chunk capped at 2048 points,
+    one full chunk and a 333-point tail that is not a multiple of the
+    contraction's 32-point step; uncapped, one 2381-point chunk) fed to the
+    FP32 engine with each contraction tile, 512 bins, against the FP64 DSSF
+    of the same trajectories (FFT form of the oracle's transform).  Bars:
+    1e-4 of the peak, and 1e-3 relative on every bin above 1e-2 of its
+    (q, component) peak; the FP64 engine fed the same points stays within
+    1e-9."""
+    if max_chunk:
+        monkeypatch.setenv("HSMD_SMD_MAX_CHUNK", str(max_chunk))
+    _say("config4 mfma dssf of a fed trajectory, gemm", gemm, "max chunk", max_chunk)
     c, sub, vac, idx, samples = _config4_dssf_case()
     T, no, dt = 2381, 512, 0.002
     models = [_model(sub[0], c, vac), _model(sub[1], c)]
@@ -564,13 +568,14 @@ def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm):
 
 
 @pytest.mark.parametrize("gemm", [4, 0, 2, 3, 1])
-def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm):
+def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm, monkeypatch):
     """The same batch integrated by both engines: FP32 RK4 with the MFMA
     contraction against FP64 RK4 with the FP64 GEMM over 2381 time points
     (dt = 0.002 keeps the two precisions' trajectories within FP32 rounding
     of each other over the run): max |a - b| <= 1e-3 of the peak, 1e-3
     relative on every bin above 1e-2 of its (q, component) peak (measured
     2.3e-6 and 1e-4), and qSS within 1e-3 of its peak."""
+    monkeypatch.setenv("HSMD_SMD_MAX_CHUNK", "2048")   # a 2048-point chunk and a 333-point tail
     _say("config4 dssf chunks gemm", gemm)
     c, sub, vac, idx, samples = _config4_dssf_case()
     T, no, dt = 2381, 512, 0.002

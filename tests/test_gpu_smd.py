@@ -124,12 +124,17 @@ def test_dssf_fp32_within_1e3_of_fp64(no):
     assert err < 1e-3
 
 
-def test_dssf_fp32_across_trajectory_chunks_within_1e3_of_fp64():
-    # 4500 time points: two full 2048-point FP32 trajectory chunks and a
-    # 404-point tail (not a multiple of the contraction's 32-point step),
-    # against the FP64 form (256-point chunks); dt = 0.002 keeps the two
-    # precisions' trajectories close over the whole run
-    _say("dssf fp32 vs fp64 across chunks", 4500)
+@pytest.mark.parametrize("max_chunk", [2048, None])
+def test_dssf_fp32_across_trajectory_chunks_within_1e3_of_fp64(max_chunk, monkeypatch):
+    # 4500 time points: with the FP32 chunk capped at 2048 points, two full
+    # chunks (the first stores the contraction, the second adds to it) and a
+    # 404-point tail (not a multiple of the contraction's 32-point step);
+    # uncapped, one 4500-point chunk; against the FP64 form (256-point
+    # chunks); dt = 0.002 keeps the two precisions' trajectories close over
+    # the whole run
+    if max_chunk:
+        monkeypatch.setenv("HSMD_SMD_MAX_CHUNK", str(max_chunk))
+    _say("dssf fp32 vs fp64 across chunks", 4500, "max chunk", max_chunk)
     c = H.Couplings(J1=1.0, J2=0.6)
     lat = H.Lattice.generate("square", 8, c)
     rng = np.random.default_rng(18)

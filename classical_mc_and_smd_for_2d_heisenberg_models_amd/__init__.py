@@ -141,6 +141,7 @@ SIGNATURES = {
     "hsmd_smd_destroy": (None, [_P]),
     "hsmd_run_mc": (_I, [ctypes.c_char_p, _I, _I, ctypes.c_uint64, _I]),
     "hsmd_run_smd": (_I, [ctypes.c_char_p, _I, _I]),
+    "hsmd_smd_window": (_I, [ctypes.c_char_p, _I, _D]),
     "hsmd_debug_philox": (_I, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, _I, _U32]),
     "hsmd_debug_exp": (_I, [_I, _D, _D]),
@@ -570,6 +571,13 @@ def run_mc(workdir, n_replicas, precision=BATCHED_F32, seed=1, strict=True):
 def run_smd(workdir, precision=BATCHED_F32, strict=True):
     """The reference's SMD program (xSMD_run) on the Settings_*.txt files of workdir."""
     _check(lib().hsmd_run_smd(os.fsencode(workdir), int(precision), int(strict)))
+
+
+def temporal_window(name, n):
+    """The SMD run's temporal window tWForm on n points (CF/smd_code.cpp:2808-2895)."""
+    out = np.zeros(int(n))
+    _check(lib().hsmd_smd_window(name.encode(), int(n), _dp(out)))
+    return out
 
 
 def debug_philox(seed, step, inst, stream, n):

@@ -34,6 +34,46 @@ void write_smd_spectra(const std::string &dir, const Lattice &L, const double *d
                        const double *qss, int ntm, double dtm, double Temp,
                        const std::string &tagDat);
 
+// Temporal windows of the DSSF time transform (CF/smd_code.cpp:2808-2895), on
+// n points: gauss exp(-80 x^2) and the sinc windows on x in [-1, 1]; Hann^p
+// and the cosine-sum windows (Nuttall, Blackman-Harris, flat top) on
+// x = i / (n - 1).  Returns false for a name it does not know (the file-read
+// "wread" is the caller's).
+static bool temporal_window(const std::string &f, int n, double *w) {
+  const double tp = 2.0 * acos(-1.0);
+  if (f == "nowin") {
+    for (int i = 0; i < n; i++) w[i] = 1.0;
+  } else if (f == "gauss") {
+    const double dx = 2.0 / (n - 1.0);
+    for (int i = 0; i < n; i++) { const double x = i * dx - 1.0; w[i] = exp(-80.0 * pow(x, 2)); }
+  } else if (f == "lancz" || f == "sincw") {
+    const double fc = f == "lancz" ? 0.5 : 1.0, dx = 2.0 / (n - 1.0);
+    for (int i = 0; i < n; i++) {
+      const double x = (i * dx - 1.0) * fc;
+      const double a = tp * x;
+      w[i] = (f == "lancz" && fabs(x) > 0.5) ? 0.0 : (a == 0.0 ? 1.0 : sin(a) / a);
+    }
+  } else if (f == "hann1" || f == "hann2" || f == "hann3") {
+    const int pw = f[4] - '0';
+    const double dx = 1.0 / (n - 1.0);
+    for (int i = 0; i < n; i++) w[i] = pow(0.5 * (1.0 - cos(tp * i * dx)), pw);
+  } else if (f == "black" || f == "ntall" || f == "flatw") {
+    const double nut[5] = {0.3635819, 0.4891775, 0.1365995, 0.0106411, 0};
+    const double har[5] = {0.35875, 0.48829, 0.14128, 0.01168, 0};
+    const double flt[5] = {0.215578950, 0.416631580, 0.277263158, 0.083578947, 0.006947368};
+    const double *a = f == "ntall" ? nut : f == "black" ? har : flt;
+    const double dx = 1.0 / (n - 1.0);
+    for (int i = 0; i < n; i++) {
+      const double x = i * dx;
+      w[i] = a[0] + (-a[1] * cos(1.0 * tp * x) + a[2] * cos(2.0 * tp * x) - a[3] * cos(3.0 * tp * x) +
+                     a[4] * cos(4.0 * tp * x));
+    }
+  } else {
+    return false;
+  }
+  return true;
+}
+
 namespace {
 
 // Settings_CodeExec.txt order (Set_Params.py code_exec_labels):
@@ -693,42 +733,15 @@ void run_smd(const std::string &dir, int precision, bool strict) {
   }
   if (!in) fail(HSMD_ERR_IO, "Error: inconsistency found in the input samples file! File : " + sf);
   const int S = (int)sample_lat.size();
-  // temporal window (CF/smd_code.cpp:2808-2895)
+  // temporal window (CF/smd_code.cpp:2808-2895); "wread" reads outBin/winvec.bin
   std::vector<double> w(ntm, 1.0);
-  {
-    const double tp = 2.0 * acos(-1.0);
-    const std::string &f = s.tWForm;
-    if (f == "wread") {
-      w = read_bin<double>(dir + "/outBin/winvec.bin", (size_t)ntm);
-      double mx = 0;
-      for (double v : w) mx = std::max(mx, v);
-      for (double &v : w) v /= mx;
-    } else if (f == "gauss") {
-      const double dx = 2.0 / (ntm - 1.0);
-      for (int i = 0; i < ntm; i++) { const double x = i * dx - 1.0; w[i] = exp(-80.0 * pow(x, 2)); }
-    } else if (f == "lancz" || f == "sincw") {
-      const double fc = f == "lancz" ? 0.5 : 1.0, dx = 2.0 / (ntm - 1.0);
-      for (int i = 0; i < ntm; i++) {
-        const double x = (i * dx - 1.0) * fc;
-        const double a = tp * x;
-        w[i] = (f == "lancz" && fabs(x) > 0.5) ? 0.0 : (a == 0.0 ? 1.0 : sin(a) / a);
-      }
-    } else if (f == "hann1" || f == "hann2" || f == "hann3") {
-      const int pw = f[4] - '0';
-      const double dx = 1.0 / (ntm - 1.0);
-      for (int i = 0; i < ntm; i++) w[i] = pow(0.5 * (1.0 - cos(tp * i * dx)), pw);
-    } else if (f == "black" || f == "ntall" || f == "flatw") {
-      const double nut[5] = {0.3635819, 0.4891775, 0.1365995, 0.0106411, 0};
-      const double har[5] = {0.35875, 0.48829, 0.14128, 0.01168, 0};
-      const double flt[5] = {0.215578950, 0.416631580, 0.277263158, 0.083578947, 0.006947368};
-      const double *a = f == "ntall" ? nut : f == "black" ? har : flt;
-      const double dx = 1.0 / (ntm - 1.0);
-      for (int i = 0; i < ntm; i++) {
-        const double x = i * dx;
-        w[i] = a[0] + (-a[1] * cos(1.0 * tp * x) + a[2] * cos(2.0 * tp * x) - a[3] * cos(3.0 * tp * x) +
-                       a[4] * cos(4.0 * tp * x));
-      }
-    }
+  if (s.tWForm == "wread") {
+    w = read_bin<double>(dir + "/outBin/winvec.bin", (size_t)ntm);
+    double mx = 0;
+    for (double v : w) mx = std::max(mx, v);
+    for (double &v : w) v /= mx;
+  } else if (!temporal_window(s.tWForm, ntm, w.data())) {
+    fail(HSMD_ERR_GUARD, "Invalid temporal window (please check inputs);");
   }
   hsmd_smd_spec sp;
   sp.precision = precision; sp.dt = s.dtm; sp.n_steps = ntm; sp.n_omega = ntm; sp.dssf = 1;
@@ -774,6 +787,19 @@ hsmd_status hsmd_run_mc(const char *workdir, int n_replicas, int precision, uint
   } catch (const std::exception &e) {
     hsmd::set_last_error(e.what());
     return HSMD_ERR_ARG;
+  }
+}
+
+hsmd_status hsmd_smd_window(const char *name, int n, double *out) {
+  try {
+    if (!name || !out) throw hsmd::Error(HSMD_ERR_ARG, "null argument");
+    if (n < 2) throw hsmd::Error(HSMD_ERR_ARG, "a window needs at least 2 points");
+    if (!hsmd::temporal_window(name, n, out))
+      throw hsmd::Error(HSMD_ERR_ARG, std::string("unknown temporal window ") + name);
+    return HSMD_OK;
+  } catch (const hsmd::Error &e) {
+    hsmd::set_last_error(e.what());
+    return e.code;
   }
 }
 

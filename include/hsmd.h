@@ -393,6 +393,13 @@ hsmd_status hsmd_run_mc(const char *workdir, int n_replicas, int precision, uint
  * writes the sample-averaged outBin/MC_Bin_SpecData_T(..)<label>.bin. */
 hsmd_status hsmd_run_smd(const char *workdir, int precision, int strict);
 
+/* Temporal window of the DSSF time transform on n >= 2 points, as the SMD
+ * run builds it from Settings' tWForm (CF/smd_code.cpp:2808-2895): "nowin",
+ * "gauss", "lancz", "sincw", "hann1", "hann2", "hann3", "black", "ntall",
+ * "flatw" ("wread", read from outBin/winvec.bin, is the run's own).  The
+ * result is the window argument of hsmd_smd_create. */
+hsmd_status hsmd_smd_window(const char *name, int n, double *out);
+
 /* --------------------------------------------------------- statistics */
 
 /* Mean and standard deviation over bins of n_values bin averages

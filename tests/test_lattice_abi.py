@@ -228,3 +228,21 @@ def test_slot_order_follows_from_positions(geom, L, pbc):
     fac, sq = _CELL[geom]
     z1 = {"square": 4, "lieb": 4, "triang": 6, "hexagn": 3, "kagome": 4}[geom]
     assert zn[0] == z1
+
+
+WINDOWS = ["nowin", "gauss", "lancz", "sincw", "hann1", "hann2", "hann3", "black", "ntall", "flatw"]
+
+
+@pytest.mark.parametrize("name", WINDOWS)
+@pytest.mark.parametrize("n", [2, 3, 256, 257, 1001])
+def test_library_windows_match_the_oracle(name, n):
+    """The windows the SMD run passes to the engine (the library's tWForm
+    table) against the oracle's restatement of CF/smd_code.cpp:2808-2895,
+    which test_oracle pins to scipy's textbook windows."""
+    np.testing.assert_allclose(H.temporal_window(name, n), cpu_ref.window(name, n), rtol=0, atol=1e-15)
+
+
+def test_library_window_errors():
+    for bad, n in [("boxcar", 16), ("wread", 16), ("hann1", 1)]:
+        with pytest.raises(H.HSMDError):
+            H.temporal_window(bad, n)

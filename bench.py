@@ -237,6 +237,9 @@ def bench_graph(timer, rng, tune, mc_steps=50):
     d = smd.dssf_q(q)
     qms = timer.stop()
     dflop = 8.0 * 3 * len(q) * N * 512 * S
+    # executed on the matrix cores: per (sample, component) block a
+    # (2 x 512 bins) x (2 x 256 phase rows) x (N rounded to 32 sites) GEMM, three split products
+    qflop = 2.0 * 1024 * 512 * ((N + 31) // 32 * 32) * 3 * 3 * S
     print("[bench] graph MC f32: %.3f ms/sweep %.3e spin-updates/s; SMD f32: %.3f ms/step "
           "%.3e site-steps/s; q-path DSSF %.1f ms" % (mms / mc_steps, mrate, sms / smd_steps, srate,
                                                      qms), file=sys.stderr, flush=True)
@@ -249,6 +252,8 @@ def bench_graph(timer, rng, tune, mc_steps=50):
                     "ms_per_step": sms / smd_steps, "n_steps_total": T, "tuning": stune},
             "dssf_qpath": {"q_points": len(q), "ms": qms,
                            "tflops": dflop / (qms * 1e-3) / 1e12,
+                           "executed_tflops": qflop / (qms * 1e-3) / 1e12,
+                           "kernel": "time_gemm_mfma_w_kernel (sites as the reduction index)",
                            "finite": bool(np.isfinite(d).all())}}
 
 

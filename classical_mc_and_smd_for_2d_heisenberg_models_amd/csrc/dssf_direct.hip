@@ -1,6 +1,7 @@
 // DSSF by direct spatial sums, for tabulated graphs (get_qct_SqData inside
 // get_dynSpectrum, CF/subrouts.cpp:11906-11912 and 8460-8525) and for any
-// list of wavevectors (a q-path).
+// list of wavevectors (a q-path): the FP64 engine's VALU kernel, and the FP32
+// engine's split-FP16 MFMA form (second half of the file).
 //
 // The time contraction Y(r, w) = sum_t w(t) S(r, t) e^{-2 pi i n t / T} is
 // already in the engine's Y buffers; this file applies
@@ -161,9 +162,88 @@ void dssf_direct_launch(const SampleDev *samples, const int *ids, int n_samples,
   HIP_CHECK(hipGetLastError());
 }
 
-template void dssf_direct_launch<float>(const SampleDev *, const int *, int, int, int, int,
-                                        const double *, const double *, const uint8_t *, float *,
-                                        const float *, double, int, double *, double *);
+// ---- FP32 q-path on the matrix cores -------------------------------------
+// For one sample s and component d, with Y_sd[r][n'] the time-contracted
+// trajectory (n' < nop: Re, n' >= nop: Im) and the phases split as
+// B[j][r] = Re e^{-i q_j.r} (j < QP) and Im e^{-i q_{j-QP}.r} (j >= QP),
+//   C_sd[n'][j] = sum_r Y_sd[r][n'] B[j][r]
+// is the time-contraction GEMM shape with the sites as the reduction index:
+// Y_sd is the FP32 operand split in-kernel, B the pre-split one, so the
+// 128 x 512 MFMA tile of the time contraction computes every (s, d) block in
+// one launch.  Then X(q, n) = C[n][q] - C[nop+n][QP+q] + i (C[nop+n][q] +
+// C[n][QP+q]) and the DSSF sums |X|^2 over the lattice's samples.
+
+// Phase table, FP16 hi/lo, chunk-major [ldk / 8][2 QP][8] over sites k (the
+// layout of the time contraction's twiddles); vacant sites, pad sites and
+// pad wavevectors are zero.  Phases in FP64: q.r reaches 10^3 rad.
+__global__ void phase_split_kernel(int nq, int QP, const double *q, int N, int ldk,
+                                   const double *pos, const uint8_t *mask, _Float16 *Bh,
+                                   _Float16 *Bl) {
+  const int J = 2 * QP;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)ldk * J) return;
+  const int k8 = (int)(e & 7);
+  const size_t jc = e >> 3;
+  const int j = (int)(jc % J), k = (int)(jc / J) * 8 + k8;
+  const int iq = j < QP ? j : j - QP;
+  float v = 0.f;
+  if (iq < nq && k < N && !mask[k]) {
+    double sn, cs;
+    sincos(q[2 * iq] * pos[2 * k] + q[2 * iq + 1] * pos[2 * k + 1], &sn, &cs);
+    v = (float)(j < QP ? cs : -sn);
+  }
+  const _Float16 h = (_Float16)v;
+  Bh[e] = h;
+  Bl[e] = (_Float16)(v - (float)h);
+}
+
+// D[d][q][n] += scale sum over the G samples of |X_sd(q, n)|^2, samples in
+// order (FP64 sums); C: G x 3 blocks of [2 nop][2 QP] FP32
+__global__ void qpath_power_kernel(int G, int nq, int QP, int nop, const float *C, double scale,
+                                   double *D) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)3 * nop * nq) return;
+  const int iq = (int)(e % nq);
+  const size_t t = e / nq;
+  const int n = (int)(t % nop), d = (int)(t / nop);
+  const size_t J = 2 * (size_t)QP, blk = (size_t)2 * nop * J;
+  double acc = 0.0;
+  for (int g = 0; g < G; g++) {
+    const float *Cb = C + (size_t)(3 * g + d) * blk;
+    const float xr = Cb[(size_t)n * J + iq] - Cb[(size_t)(nop + n) * J + QP + iq];
+    const float xi = Cb[(size_t)(nop + n) * J + iq] + Cb[(size_t)n * J + QP + iq];
+    acc += (double)xr * (double)xr + (double)xi * (double)xi;
+  }
+  D[((size_t)d * nq + iq) * nop + n] += scale * acc;
+}
+
+void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const float *A,
+                             const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
+                             float *Y, hipStream_t st);
+
+// qpath_phase_launch fills Bh, Bl (2 QP x ldk halves each, ldk = N rounded up
+// to 32) for nq <= QP wavevectors; qpath_mfma_launch then runs G samples:
+// samples_dev holds G x 3 GEMM blocks (M = 2 nop rows of Y_sd, row stride
+// 2 nop, outputs [2 nop][2 QP] consecutive in C) and D accumulates the powers.
+void qpath_phase_launch(int nq, int QP, const double *q, int N, const double *pos,
+                        const uint8_t *mask, _Float16 *Bh, _Float16 *Bl) {
+  const int ldk = ((N + 31) / 32) * 32;
+  const size_t nb = (size_t)ldk * 2 * QP;
+  hipLaunchKernelGGL(phase_split_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, 0, nq, QP,
+                     q, N, ldk, pos, mask, Bh, Bl);
+  HIP_CHECK(hipGetLastError());
+}
+
+void qpath_mfma_launch(int nq, int QP, int N, const _Float16 *Bh, const _Float16 *Bl,
+                       const void *samples_dev, int G, int nop, const float *Y, float *C,
+                       double scale, double *D) {
+  launch_time_gemm_mfma_w(samples_dev, 3 * G, 2 * nop, Y, Bh, Bl, 0, N, 2 * QP, C, 0);
+  const size_t n = (size_t)3 * nop * nq;
+  hipLaunchKernelGGL(qpath_power_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, G, nq,
+                     QP, nop, (const float *)C, scale, D);
+  HIP_CHECK(hipGetLastError());
+}
+
 template void dssf_direct_launch<double>(const SampleDev *, const int *, int, int, int, int,
                                          const double *, const double *, const uint8_t *, double *,
                                          const double *, double, int, double *, double *);

@@ -848,6 +848,75 @@ void dssf_direct_launch(const SampleDev *samples, const int *ids, int n_samples,
 // DSSF at wavevectors q (nq x 2) by direct spatial sums over the sites of
 // lattice li (vacant sites excluded): out[3][nq][n_omega], sample-summed
 // |X|^2 / (N T).
+void qpath_phase_launch(int nq, int QP, const double *q, int N, const double *pos,
+                        const uint8_t *mask, _Float16 *Bh, _Float16 *Bl);
+void qpath_mfma_launch(int nq, int QP, int N, const _Float16 *Bh, const _Float16 *Bl,
+                       const void *samples_dev, int G, int nop, const float *Y, float *C,
+                       double scale, double *D);
+
+// FP32 q-path DSSF on the matrix cores: wavevectors in chunks of 256 (the
+// phase table's 2 x 256 rows are the contraction tile's 512 columns), samples
+// in groups whose GEMM outputs fit in 4 GB.  Same sums as the direct kernel:
+// X_s(q, w) = sum_r e^{-i q.r} Y_s(r, w), D = sum_s |X_s|^2 / (N T).
+static void dssf_qpath_mfma(SMDEngine &e, int li, int nq, const double *q, double *out) {
+  const Lattice &L = e.lats[li];
+  if (L.rvec.size() != (size_t)2 * L.N) throw Error(HSMD_ERR_ARG, "lattice has no site positions");
+  const int N = L.N, no = e.nop, nkeep = e.spec.n_omega;
+  constexpr int QP = 256;
+  std::vector<int> ids;
+  for (int s = 0; s < e.S; s++) if (e.sample_lat[s] == li) ids.push_back(s);
+  std::vector<double> pos((size_t)2 * N);
+  std::vector<uint8_t> mask(N);
+  for (int p = 0; p < N; p++) {
+    const int i = L.perm[p];
+    pos[2 * p] = L.rvec[2 * (size_t)i];
+    pos[2 * p + 1] = L.rvec[2 * (size_t)i + 1];
+    mask[p] = L.vacant.empty() ? 0 : (uint8_t)(L.vacant[i] != 0);
+  }
+  DevBuf<double> dpos, dq, d;
+  DevBuf<uint8_t> dmask;
+  dpos.upload(pos.data(), pos.size());
+  dmask.upload(mask.data(), mask.size());
+  const int ldk = ((N + 31) / 32) * 32;
+  DevBuf<_Float16> bh, bl;
+  bh.alloc((size_t)ldk * 2 * QP);
+  bl.alloc((size_t)ldk * 2 * QP);
+  const size_t blk = (size_t)2 * no * 2 * QP;   // one (sample, component) output block
+  const int nids = (int)ids.size();
+  const int G = std::max(1, std::min(std::max(1, nids), (int)((1ull << 32) / (3 * blk * sizeof(float)))));
+  DevBuf<float> C;
+  C.alloc((size_t)3 * G * blk);
+  DevBuf<GemmSampleHost> gsd;
+  gsd.alloc((size_t)3 * G);
+  d.alloc((size_t)3 * QP * no);
+  const double scale = 1.0 / ((double)N * (double)e.spec.n_steps);
+  const long long ldy = 2LL * no;
+  std::vector<double> h((size_t)3 * QP * no);
+  std::vector<GemmSampleHost> gs((size_t)3 * G);
+  for (int q0 = 0; q0 < nq; q0 += QP) {
+    const int nc = std::min(QP, nq - q0);
+    d.zero();
+    dq.upload(q + 2 * (size_t)q0, (size_t)2 * nc);
+    qpath_phase_launch(nc, QP, dq.p, N, dpos.p, dmask.p, bh.p, bl.p);
+    for (int g0 = 0; g0 < nids; g0 += G) {
+      const int ng = std::min(G, nids - g0);
+      for (int g = 0; g < ng; g++)
+        for (int c = 0; c < 3; c++)
+          gs[3 * g + c] = {2 * no, 2 * no, e.hs[ids[g0 + g]].y_off + (long long)c * N * ldy,
+                           (long long)(3 * g + c) * (long long)blk};
+      HIP_CHECK(hipMemcpy(gsd.p, gs.data(), (size_t)3 * ng * sizeof(GemmSampleHost),
+                          hipMemcpyHostToDevice));
+      qpath_mfma_launch(nc, QP, N, bh.p, bl.p, gsd.p, ng, no, (const float *)e.ybuf.p, C.p, scale,
+                        d.p);
+    }
+    HIP_CHECK(hipMemcpy(h.data(), d.p, (size_t)3 * nc * no * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; c++)
+      for (int k = 0; k < nc; k++)
+        for (int n = 0; n < nkeep; n++)
+          out[((size_t)c * nq + q0 + k) * nkeep + n] = h[((size_t)c * nc + k) * no + n];
+  }
+}
+
 template <typename T>
 static void dssf_direct(SMDEngine &e, int li, int nq, const double *q, double *out) {
   const Lattice &L = e.lats[li];
@@ -955,7 +1024,7 @@ void smd_dssf(SMDEngine &e, int li, double *out, double *qss) {
   if (L.tabulated) {
     // get_dynSpectrum takes get_qct_SqData for tabulated graphs (CF/subrouts.cpp:11906-11908)
     const std::vector<double> q = tabulated_q_grid(L);
-    if (e.f32) dssf_direct<float>(e, li, L.Nq, q.data(), out);
+    if (e.f32) dssf_qpath_mfma(e, li, L.Nq, q.data(), out);
     else dssf_direct<double>(e, li, L.Nq, q.data(), out);
   } else {
     if (e.f32) dssf_space<float>(e, li, out); else dssf_space<double>(e, li, out);
@@ -978,7 +1047,7 @@ void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out) {
   if (!e.spec.dssf) throw Error(HSMD_ERR_STATE, "DSSF accumulation is off");
   if (e.t_next != e.spec.n_steps) throw Error(HSMD_ERR_STATE, "DSSF needs all n_steps time points");
   flush_chunk(e);
-  if (e.f32) dssf_direct<float>(e, li, nq, q, out); else dssf_direct<double>(e, li, nq, q, out);
+  if (e.f32) dssf_qpath_mfma(e, li, nq, q, out); else dssf_direct<double>(e, li, nq, q, out);
   HIP_CHECK(hipDeviceSynchronize());
 }
 

@@ -715,6 +715,41 @@ def test_config5_graph_fp64_replay(prec):
     _say("max |dS| %.2e" % worst)
 
 
+@pytest.mark.parametrize("nq", [200, 300])
+def test_config5_qpath_mfma_dssf_of_a_known_trajectory(nq):
+    """The FP32 engine's split-FP16 MFMA q-path (sites as the reduction
+    index, wavevectors in chunks of 256) on the oracle's own FP64
+    trajectories of the graph (four samples, 256 time points, 5 % vacancies)
+    fed to the engine, against the oracle's direct sums along the path
+    (200 points; 300 spans two wavevector chunks).  Bars: 1e-4 of the peak
+    and 1e-3 relative on every bin above 1e-2 of its (q, component) peak;
+    the FP64 engine fed the same points within 1e-9."""
+    _say("config5 q-path mfma of a fed trajectory, nq", nq)
+    c, lat, vac = _graph()
+    m = _model(lat, c, vac)
+    rng = np.random.default_rng(509)
+    S, T, no, dt = 4, 256, 128, 0.01
+    samples = [_unit(rng, (lat.N, 3)) for _ in range(S)]
+    trajs = _pmap(lambda s0: m.smd_run(s0, dt, T, gauss_seidel=False)[1], samples, workers=4)
+    q = _q_path(nq, qmax=3 * math.pi)
+    rv = lat.positions()
+    ref = sum(cpu_ref.dssf_direct_fft(tr, rv, q, np.ones(T), no, mask=vac) for tr in trajs)
+    for prec in (H.BATCHED_F32, H.BATCHED_F64):
+        smd = H.SpinDynamics([lat], np.zeros(S, np.int32), dt, T, n_omega=no, precision=prec)
+        _feed_blocks(smd, trajs, step=64)
+        d = smd.dssf_q(q)
+        err = np.abs(d - ref).max() / ref.max()
+        pk = ref.max(axis=2, keepdims=True)
+        big = ref > 1e-2 * pk
+        rel = (np.abs(d - ref)[big] / ref[big]).max()
+        _say("prec %d: max err / peak %.2e, max rel err above 1e-2 peak %.2e" % (prec, err, rel))
+        if prec == H.BATCHED_F32:
+            assert err < 1e-4
+            assert rel < 1e-3
+        else:
+            assert err < 1e-9
+
+
 @pytest.mark.parametrize("prec,tol", [(H.BATCHED_F32, 3e-5), (H.BATCHED_F64, 1e-10)])
 def test_config5_graph_smd_and_qpath_dssf_match_oracle(prec, tol):
     """SMD on the graph in the global-memory form (autotuned shape for

@@ -42,8 +42,11 @@ HSMD_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = M0 * c.x, hi0 = mulhi32(M0, c.x);
-    const uint32_t lo1 = M1 * c.z, hi1 = mulhi32(M1, c.z);
+    // one 32 x 32 -> 64-bit product per multiplier (a single v_mad_u64_u32
+    // on the GPU instead of separate low and high multiplies)
+    const uint64_t p0 = (uint64_t)M0 * (uint64_t)c.x, p1 = (uint64_t)M1 * (uint64_t)c.z;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     u32x4 n;
     n.x = hi1 ^ c.y ^ k0;
     n.y = lo1;

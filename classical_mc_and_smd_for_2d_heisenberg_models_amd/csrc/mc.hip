@@ -621,9 +621,13 @@ void heis_sweep_fast(const SweepArgs a) {
       __builtin_amdgcn_make_buffer_rsrc((void *)a.nb16, (short)0, 0x7fffffff, 0x00020000);
   const __amdgpu_buffer_rsrc_t prsrc =
       __builtin_amdgcn_make_buffer_rsrc((void *)a.perm, (short)0, 0x7fffffff, 0x00020000);
-  auto fetch = [&](int col, uint32_t (&q)[ZH], int &lab, bool &v) {
-    const int p = a.col_off[col] + tid;
-    v = p < a.col_off[col + 1];
+  // class bounds: the current class's (cb, ce) and the next one's (cbn, cen)
+  // travel with the labels; the class after that is read one iteration
+  // ahead, so no scalar load is waited for at the top of an iteration
+  int cb = 0, ce = 0, cbn = 0, cen = 0;
+  auto fetch = [&](uint32_t (&q)[ZH], int &lab, bool &v, int b0, int b1) {
+    const int p = b0 + tid;
+    v = p < b1;
     const int pc4 = 4 * (v ? p : 0);
 #pragma unroll
     for (int k = 0; k < ZH; k++) q[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, pc4, 4 * k * N, 0);
@@ -631,7 +635,13 @@ void heis_sweep_fast(const SweepArgs a) {
   };
   if (total > 0) {
     bool va;
-    fetch(0, qa, la, va);
+    cb = a.col_off[0];
+    ce = a.col_off[1];
+    fetch(qa, la, va, cb, ce);
+    if (total > 1) {
+      cbn = a.col_off[1 % nc];
+      cen = a.col_off[1 % nc + 1];
+    }
 #pragma unroll
     for (int k = 0; k < ZH; k++) qa[k] = va ? qa[k] : pad;
   }
@@ -639,11 +649,17 @@ void heis_sweep_fast(const SweepArgs a) {
   for (int it = 0; it < total; it++) {
     const int pass = it / nc;
     const int col = it - pass * nc;
-    if (it + 1 < total) fetch((it + 1) % nc, qb, lb, vb);
-    const int p = a.col_off[col] + tid;
+    if (it + 1 < total) fetch(qb, lb, vb, cbn, cen);
+    int cb2 = 0, ce2 = 0;
+    if (it + 2 < total) {
+      const int c2 = (it + 2) % nc;
+      cb2 = a.col_off[c2];
+      ce2 = a.col_off[c2 + 1];
+    }
+    const int p = cb + tid;
     // compact labels: the words any site of this wave's 64-site chunk uses
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
-    if (p < a.col_off[col + 1]) {
+    if (p < ce) {
       T hx = T(0), hy = T(0), hz = hz0;
       if constexpr (!CP) {
         // the neighbour reads in groups (10 neighbours in FP32, 4 in FP64),
@@ -720,7 +736,7 @@ void heis_sweep_fast(const SweepArgs a) {
       if constexpr (AOS) s4[p] = make_float4(x, y, z, 0.f);
       else { sx[p] = x; sy[p] = y; sz[p] = z; }
     }
-    if (pass >= nor) keep = !(col & 1) && p < a.col_off[col + 1];
+    if (pass >= nor) keep = !(col & 1) && p < ce;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ZH; k++) {
@@ -731,6 +747,10 @@ void heis_sweep_fast(const SweepArgs a) {
     }
     la = lb;
     asm volatile("" : "+v"(la));
+    cb = cbn;
+    ce = cen;
+    cbn = cb2;
+    cen = ce2;
   }
   if (a.table_energy) {
     // E = escale sum_i S_i . sum_{s in emask} (J_s S_nb(i,s)) - h sum_i z_i,

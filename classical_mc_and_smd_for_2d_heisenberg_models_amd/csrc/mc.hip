@@ -601,7 +601,6 @@ void heis_sweep_fast(const SweepArgs a) {
   const int nc = a.n_colours;
   const int total = (nor + nhb) * nc;
   const int tid = threadIdx.x;
-  const uint32_t pad = (uint32_t)N | ((uint32_t)N << 16);
   const T hz0 = a.cp.h_on ? T(-a.cp.h) : T(0);
   const uint32_t sweep16 = sweep_of(a) * 16u;   // read once, before the label loads
   // FP32 heat bath: Philox words z, w kept for the next (odd) class
@@ -609,12 +608,10 @@ void heis_sweep_fast(const SweepArgs a) {
   bool keep = false;
   uint32_t qa[ZH], qb[ZH];
   int la = 0, lb = 0;
-  bool vb = false;
   // the packed labels are padded to ZC slots with the zero-spin sentinel
   // (coupling 0), so every slot is read without a branch.  The next class's
-  // labels are loaded unconditionally (lanes past the class read row 0) and
-  // replaced by the sentinel only after the barrier, so the loads stay in
-  // flight across the current class's update
+  // labels are loaded unconditionally (lanes past the class read row 0), so
+  // the loads stay in flight across the current class's update
   // buffer loads: the lane's site is the per-lane offset, the word row the
   // scalar offset, so the loads need no per-lane address arithmetic
   const __amdgpu_buffer_rsrc_t nrsrc =
@@ -642,14 +639,17 @@ void heis_sweep_fast(const SweepArgs a) {
       cbn = a.col_off[1 % nc];
       cen = a.col_off[1 % nc + 1];
     }
-#pragma unroll
-    for (int k = 0; k < ZH; k++) qa[k] = va ? qa[k] : pad;
   }
   __syncthreads();
-  for (int it = 0; it < total; it++) {
+  // one colour class; the label registers alternate between two sets (the
+  // loop runs two classes per trip), so no copies follow the barrier.  Lanes
+  // past a class hold whatever row 0's labels were loaded and do no update.
+  auto iteration = [&](int it, const uint32_t (&qc)[ZH], int lc, uint32_t (&qn)[ZH], int &ln) {
     const int pass = it / nc;
     const int col = it - pass * nc;
-    if (it + 1 < total) fetch(qb, lb, vb, cbn, cen);
+    bool vn = false;
+    if (it + 1 < total) fetch(qn, ln, vn, cbn, cen);
+    (void)vn;
     int cb2 = 0, ce2 = 0;
     if (it + 2 < total) {
       const int c2 = (it + 2) % nc;
@@ -661,6 +661,28 @@ void heis_sweep_fast(const SweepArgs a) {
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
     if (p < ce) {
       T hx = T(0), hy = T(0), hz = hz0;
+      // the site's own spin is read with the first group of neighbours, so
+      // its latency hides behind the field sums
+      T x, y, z;
+      get(p, x, y, z);
+      // heat-bath draws: independent of the field, so they are computed
+      // while the first group of neighbour reads is in flight
+      const uint32_t step = sweep16 + (uint32_t)pass;
+      uint32_t w0 = keep_z, w1 = keep_w;
+      u32x4 r64 = {0u, 0u, 0u, 0u};
+      auto draw_early = [&]() {
+        if (pass < nor) return;
+        if constexpr (sizeof(T) == 4) {
+          if (!((col & 1) && keep)) {
+            const u32x4 r = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
+            w0 = r.x; w1 = r.y;
+            keep_z = r.z; keep_w = r.w;
+          }
+        } else {
+          r64 = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
+        }
+      };
+      if constexpr (CP) draw_early();
       if constexpr (!CP) {
         // the neighbour reads in groups (10 neighbours in FP32, 4 in FP64),
         // each group issued before its first product (the scheduling
@@ -674,10 +696,11 @@ void heis_sweep_fast(const SweepArgs a) {
 #pragma unroll
           for (int k = 0; k < GW; k++) {
             if (k0 + k < ZH) {
-              nget(qa[k0 + k], 0, nx[2 * k], ny[2 * k], nz[2 * k]);
-              if (2 * (k0 + k) + 1 < ZC) nget(qa[k0 + k], 1, nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
+              nget(qc[k0 + k], 0, nx[2 * k], ny[2 * k], nz[2 * k]);
+              if (2 * (k0 + k) + 1 < ZC) nget(qc[k0 + k], 1, nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
             }
           }
+          if (k0 == 0) draw_early();
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int u = 0; u < 2 * GW; u++) {
@@ -696,15 +719,15 @@ void heis_sweep_fast(const SweepArgs a) {
         int i0, i1;
         T J0, Jz0, J1, Jz1;
         if constexpr (CP) {
-          const uint32_t w = qa[k];
+          const uint32_t w = qc[k];
           i0 = (int)(w & 0x3fffu);
           i1 = (int)((w >> 16) & 0x3fffu);
           const int s0 = (int)((w >> 14) & 3u), s1 = (int)(w >> 30);
           J0 = shell_j<T>(a, s0, false); Jz0 = shell_j<T>(a, s0, true);
           J1 = shell_j<T>(a, s1, false); Jz1 = shell_j<T>(a, s1, true);
         } else {
-          i0 = (int)(qa[k] & 0xffffu);
-          i1 = (int)(qa[k] >> 16);
+          i0 = (int)(qc[k] & 0xffffu);
+          i1 = (int)(qc[k] >> 16);
           J0 = cj_xy<T>(a.cp, 2 * k); Jz0 = cj_z<T>(a.cp, 2 * k);
           J1 = cj_xy<T>(a.cp, 2 * k + 1); Jz1 = cj_z<T>(a.cp, 2 * k + 1);
         }
@@ -714,43 +737,25 @@ void heis_sweep_fast(const SweepArgs a) {
         hx += J0 * x0; hy += J0 * y0; hz += Jz0 * z0;
         hx += J1 * x1; hy += J1 * y1; hz += Jz1 * z1;
       }
-      T x, y, z;
-      get(p, x, y, z);
       if (pass < nor) {
         overrelax<T>(hx, hy, hz, x, y, z);
       } else {
-        const uint32_t step = sweep16 + (uint32_t)pass;
-        if constexpr (sizeof(T) == 4) {
-          uint32_t w0 = keep_z, w1 = keep_w;
-          if (!((col & 1) && keep)) {
-            const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
-            w0 = r.x; w1 = r.y;
-            keep_z = r.z; keep_w = r.w;
-          }
-          heat_bath_f32(fbeta, hx, hy, hz, w0, w1, x, y, z);
-        } else {
-          const u32x4 r = draw4(a.seed, (uint32_t)la, step, (uint32_t)c, STREAM_MC_UPDATE);
-          heat_bath<T>(beta, hx, hy, hz, r, x, y, z);
-        }
+        if constexpr (sizeof(T) == 4) heat_bath_f32(fbeta, hx, hy, hz, w0, w1, x, y, z);
+        else heat_bath<T>(beta, hx, hy, hz, r64, x, y, z);
       }
       if constexpr (AOS) s4[p] = make_float4(x, y, z, 0.f);
       else { sx[p] = x; sy[p] = y; sz[p] = z; }
     }
     if (pass >= nor) keep = !(col & 1) && p < ce;
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < ZH; k++) {
-      qa[k] = vb ? qb[k] : pad;
-      // settled here: no later wait on the memory counter (which would also
-      // cover the next class's loads) is needed for these registers
-      asm volatile("" : "+v"(qa[k]));
-    }
-    la = lb;
-    asm volatile("" : "+v"(la));
     cb = cbn;
     ce = cen;
     cbn = cb2;
     cen = ce2;
+  };
+  for (int it = 0; it < total; it += 2) {
+    iteration(it, qa, la, qb, lb);
+    if (it + 1 < total) iteration(it + 1, qb, lb, qa, la);
   }
   if (a.table_energy) {
     // E = escale sum_i S_i . sum_{s in emask} (J_s S_nb(i,s)) - h sum_i z_i,

@@ -413,18 +413,22 @@ SIMDS = 1024
 VALU_PEAK_GINST = SIMDS * 2.4e9 / 4 / 1e9
 
 
-def valu_roofline(profile, kernel_s, hbm_alg_bytes, kernel, per=1.0, extra=None,
+def valu_roofline(profile, kernel_s, hbm_alg_bytes, kernel, unit="launch", extra=None,
                   traffic_profile=None):
     """Issue-bound roofline of an LDS-resident kernel: VALU instructions per
-    launch (SQ_INSTS_VALU from the committed PMC summary) over the live launch
-    time against the SIMDs' issue peak, with the HBM view beside it
-    (algorithmic HBM bytes per launch over the same time, and the PMC HBM
-    bytes per launch).  per: launches the profile's counts cover per live
-    time unit (1 for a whole launch)."""
-    vi = profile_value(profile, "valu_insts_per_launch")
-    traffic = profile_value(traffic_profile or profile, "traffic_bytes_per_launch")
+    unit (SQ_INSTS_VALU from the committed PMC summary) over the live time of
+    one unit against the SIMDs' issue peak, with the HBM view beside it
+    (algorithmic HBM bytes per unit over the same time, and the PMC HBM bytes
+    per unit).  unit: "launch" (one sweep launch: kernel_s is a launch's
+    duration) or "step" (one RK4 step of a persistent launch: kernel_s is a
+    launch's duration divided by its steps; the profile's counts are divided
+    by the steps of the profiled launch), so the two need not run equally
+    many steps per launch."""
+    vi = profile_value(profile, "valu_insts_per_" + unit)
+    traffic = profile_value(traffic_profile or profile, "traffic_bytes_per_" + unit)
     lds = profile_value(profile, "lds_wait_frac")
-    ach = None if vi is None else vi * per / kernel_s / 1e9
+    per = 1.0
+    ach = None if vi is None else vi / kernel_s / 1e9
     hbm_gbs = hbm_alg_bytes / kernel_s / 1e9
     out = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
            "frac": None if ach is None else ach / VALU_PEAK_GINST,
@@ -632,7 +636,7 @@ def bench_smd(timer, args, rng, tune):
     gemm = smd.tuning["gemm"] if smd.tuning else 4
     roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
                          SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % nb,
-                         per=1.0 / nb,
+                         unit="step",
                          extra={"per": "RK4 step of the 512-sample batch",
                                 "gather_model": {"bytes_per_step": rk_gather,
                                                  "achieved": rk_gather / (rk_ms * 1e-3) / 1e9,

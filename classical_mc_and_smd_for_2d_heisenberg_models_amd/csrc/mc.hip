@@ -504,13 +504,19 @@ template <typename T>
 __device__ __forceinline__ T shell_j(const SweepArgs &a, int sh, bool z);
 template <>
 __device__ __forceinline__ float shell_j<float>(const SweepArgs &a, int sh, bool z) {
-  const float *j = z ? a.fshjz : a.fshj;
-  return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
+  // selects values, not pointers: a pointer into the kernel's argument
+  // block makes the compiler copy the whole block to scratch memory
+  const float j0 = z ? a.fshjz[0] : a.fshj[0];
+  const float j1 = z ? a.fshjz[1] : a.fshj[1];
+  const float j2 = z ? a.fshjz[2] : a.fshj[2];
+  return sh == 0 ? j0 : (sh == 1 ? j1 : j2);
 }
 template <>
 __device__ __forceinline__ double shell_j<double>(const SweepArgs &a, int sh, bool z) {
-  const double *j = z ? a.shjz : a.shj;
-  return sh == 0 ? j[0] : (sh == 1 ? j[1] : j[2]);
+  const double j0 = z ? a.shjz[0] : a.shj[0];
+  const double j1 = z ? a.shjz[1] : a.shj[1];
+  const double j2 = z ? a.shjz[2] : a.shj[2];
+  return sh == 0 ? j0 : (sh == 1 ? j1 : j2);
 }
 
 // Sites of the statically placed float4 replica (STAT kernels): 150 KB.
@@ -622,7 +628,7 @@ void heis_sweep_fast(const SweepArgs a) {
   // travel with the labels; the class after that is read one iteration
   // ahead, so no scalar load is waited for at the top of an iteration
   int cb = 0, ce = 0, cbn = 0, cen = 0;
-  auto fetch = [&](uint32_t (&q)[ZH], int &lab, bool &v, int b0, int b1) {
+  auto fetch = [&](uint32_t (&q)[ZH], int &lab, bool &v, int b0, int b1) __attribute__((always_inline)) {
     const int p = b0 + tid;
     v = p < b1;
     const int pc4 = 4 * (v ? p : 0);
@@ -644,7 +650,10 @@ void heis_sweep_fast(const SweepArgs a) {
   // one colour class; the label registers alternate between two sets (the
   // loop runs two classes per trip), so no copies follow the barrier.  Lanes
   // past a class hold whatever row 0's labels were loaded and do no update.
-  auto iteration = [&](int it, const uint32_t (&qc)[ZH], int lc, uint32_t (&qn)[ZH], int &ln) {
+  // always inlined: called twice, a lambda that stays a call takes the label
+  // arrays by reference in scratch memory
+  auto iteration = [&](int it, const uint32_t (&qc)[ZH], int lc, uint32_t (&qn)[ZH],
+                       int &ln) __attribute__((always_inline)) {
     const int pass = it / nc;
     const int col = it - pass * nc;
     bool vn = false;
@@ -661,16 +670,19 @@ void heis_sweep_fast(const SweepArgs a) {
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
     if (p < ce) {
       T hx = T(0), hy = T(0), hz = hz0;
-      // the site's own spin is read with the first group of neighbours, so
-      // its latency hides behind the field sums
+      // FP32: the site's own spin is read with the first group of
+      // neighbours, so its latency hides behind the field sums, and the
+      // heat-bath draws (independent of the field) are computed while the
+      // first group of neighbour reads is in flight.  FP64 does both after
+      // the sums: held across them, they would push its 128 VGPRs into
+      // scratch memory.
+      constexpr bool EARLY = sizeof(T) == 4;
       T x, y, z;
-      get(p, x, y, z);
-      // heat-bath draws: independent of the field, so they are computed
-      // while the first group of neighbour reads is in flight
+      if constexpr (EARLY) get(p, x, y, z);
       const uint32_t step = sweep16 + (uint32_t)pass;
       uint32_t w0 = keep_z, w1 = keep_w;
       u32x4 r64 = {0u, 0u, 0u, 0u};
-      auto draw_early = [&]() {
+      auto draw_early = [&]() __attribute__((always_inline)) {
         if (pass < nor) return;
         if constexpr (sizeof(T) == 4) {
           if (!((col & 1) && keep)) {
@@ -682,7 +694,7 @@ void heis_sweep_fast(const SweepArgs a) {
           r64 = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
         }
       };
-      if constexpr (CP) draw_early();
+      if constexpr (CP && EARLY) draw_early();
       if constexpr (!CP) {
         // the neighbour reads in groups (10 neighbours in FP32, 4 in FP64),
         // each group issued before its first product (the scheduling
@@ -700,7 +712,9 @@ void heis_sweep_fast(const SweepArgs a) {
               if (2 * (k0 + k) + 1 < ZC) nget(qc[k0 + k], 1, nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
             }
           }
-          if (k0 == 0) draw_early();
+          if constexpr (EARLY) {
+            if (k0 == 0) draw_early();
+          }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int u = 0; u < 2 * GW; u++) {
@@ -736,6 +750,10 @@ void heis_sweep_fast(const SweepArgs a) {
         get(i1, x1, y1, z1);
         hx += J0 * x0; hy += J0 * y0; hz += Jz0 * z0;
         hx += J1 * x1; hy += J1 * y1; hz += Jz1 * z1;
+      }
+      if constexpr (!EARLY) {
+        get(p, x, y, z);
+        draw_early();
       }
       if (pass < nor) {
         overrelax<T>(hx, hy, hz, x, y, z);

@@ -688,7 +688,9 @@ void orc_mc_reference(void *mp, int model, int R, const double *temps, int n_or,
 #endif
   for (int s = sweep0; s < sweep0 + n_sweeps; s++) {
     for (int k = 0; k < R; k++) soc[cos[k]] = k;
-#pragma omp parallel for schedule(static)
+    // small chains run on the calling thread: a parallel region per sweep
+    // costs more than a sweep of a few dozen sites
+#pragma omp parallel for schedule(static) if ((long long)R * N >= 2048)
     for (int c = 0; c < R; c++) {
       double *S = spins + (size_t)c * N * 3;
       const int slot = soc[c];

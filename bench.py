@@ -58,7 +58,7 @@ SMD_CHUNK = 8192
 # DSSF time-contraction kernel per contraction tile
 GEMM_KERNEL = {0: "time_gemm_mfma_q_kernel", 1: "time_gemm_mfma_kernel",
                2: "time_gemm_mfma_q256_kernel", 3: "time_gemm_mfma_db_kernel",
-               4: "time_gemm_mfma_w_kernel"}
+               4: "time_gemm_mfma_w_kernel", 5: "time_gemm_mfma_p_kernel"}
 # RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
 SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
                    3: "smd_kernel_wide<float,12,false>"}

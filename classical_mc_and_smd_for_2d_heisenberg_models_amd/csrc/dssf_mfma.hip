@@ -560,6 +560,182 @@ void time_gemm_mfma_w_kernel(const GemmSample *samples, const float *__restrict_
     }
 }
 
+// Pipelined form of the 128 x 512 tile: four waves, one per SIMD, each
+// owning all 128 rows and 128 of the columns (8 x 8 MFMA blocks, 256
+// accumulators).  The twiddles a wave multiplies are its own (no other wave
+// uses its columns), so they skip LDS: each wave loads its FP16 hi / lo
+// fragments straight from the chunk-major table into registers (16-byte
+// loads, 256 contiguous bytes per 16-lane group), one 32-point step ahead.
+// Only the trajectory tile, which all four waves share, passes through LDS,
+// in a ring of three 16 KB stages: during step k the waves multiply step k's
+// fragments (in registers), refill each row block's fragments from stage
+// k + 1 as soon as its products are issued, and split / store step k + 2's
+// trajectory values (loaded one step earlier) into the third stage; one
+// barrier per step.  LDS traffic per step and CU is 64 KB of reads and 16 KB
+// of writes instead of the 192 KB + 80 KB of the single-stage tile.  The
+// products, their order per accumulator and the Y update are those of the
+// 128 x 512 kernel above, so the two give identical sums.  Measured on
+// MI355X: 50.3 ms per 2048-point chunk of the 512-sample batch against
+// 49.8 ms for the single-stage tile (a copy with the twiddles staged in LDS
+// by direct-to-LDS loads over two 80 KB stages: 52 ms).  In a probe of the
+// same instruction stream the matrix instructions alone ran at 2.14 PFLOP/s,
+// and 28 % faster than the full kernel when every load hit the cache: with
+// one wave per SIMD a one-step-ahead prefetch does not cover the loads'
+// latency.  The autotuner times it beside the other tiles.
+constexpr int P_THREADS = 256;
+constexpr int P_AST = 2 * 4 * QM * 8;     // halves of a trajectory stage [hi, lo][4 chunks][128 rows][8]
+
+__global__ __launch_bounds__(P_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void time_gemm_mfma_p_kernel(const GemmSample *samples, const float *__restrict__ A,
+                             const _Float16 *__restrict__ B8h, const _Float16 *__restrict__ B8l,
+                             int k0, int K, int ldb, float *Y) {
+  __shared__ __align__(16) _Float16 As[3 * P_AST];
+  const GemmSample sd = samples[blockIdx.z];
+  const int M = sd.M, lda = sd.lda;
+  const int nj = ldb / W_TJ;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int m0 = ((slot / nj) * 8 + xcd) * QM, j0 = (slot % nj) * W_TJ;
+  if (m0 >= M) return;
+  const float *Ab = A + sd.a_off;
+  float *Yb = Y + sd.y_off;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wj = wave * 128;
+  const int li = lane & 15, lg = lane >> 4;
+  // trajectory staging: thread t owns row am = t % 128, time points 16 kh .. 16 kh + 15
+  const int am = tid & 127;
+  const int kh = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const __amdgpu_buffer_rsrc_t arsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)Ab, (short)0, 0x7fffffff, 0x00020000);
+  const int acol = 4 * min(m0 + am, lda - 1);
+  const int lda4 = 4 * lda;
+  // twiddle fragments: lane (li, lg) of column block b reads table row
+  // (k0 + 32 st) / 8 + lg, column j0 + wj + 16 b + li: the per-lane part of
+  // the byte offset is fixed, the step's row the scalar part, the block the
+  // instruction's immediate offset
+  const __amdgpu_buffer_rsrc_t hrsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)B8h, (short)0, 0x7fffffff, 0x00020000);
+  const __amdgpu_buffer_rsrc_t lrsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)B8l, (short)0, 0x7fffffff, 0x00020000);
+  const int bvoff = lg * ldb * 16 + (j0 + wj + li) * 16;
+  const int nsteps = (K + WK - 1) / WK;
+  float4_t acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) acc[a][b] = (float4_t){0.f, 0.f, 0.f, 0.f};
+
+  // every step issues the same loads (so the compiler's wait counts are
+  // exact): time rows past the chunk are clamped to its last row, and their
+  // values are zeroed when split (lim: the number of valid rows); the time
+  // index is uniform over the wave, so the condition is scalar (rows past M
+  // are clamped into the row stride, computed and never stored)
+  auto load_a = [&](float (&ra)[16], int &lim, int st) __attribute__((always_inline)) {
+    const int k1 = st * WK + 16 * kh;
+    lim = K - k1;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const int r = min(k1 + e, K - 1) * lda4;
+      ra[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(arsrc, acol, r, 0));
+    }
+  };
+  auto store_a = [&](const float (&ra)[16], int lim, _Float16 *as) __attribute__((always_inline)) {
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      half8_t hv, lv;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        // hi: the value cut to FP16's 11 significant bits (exact in FP16 for
+        // |x| >= 2^-14); lo: the exact FP32 remainder, rounded to FP16
+        const float x = 8 * g + e < lim ? ra[8 * g + e] : 0.f;
+        const float h = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) & 0xffffe000u);
+        hv[e] = (_Float16)h;
+        lv[e] = (_Float16)(x - h);
+      }
+      *(half8_t *)&as[((2 * kh + g) * QM + am) * 8] = hv;
+      *(half8_t *)&as[((4 + 2 * kh + g) * QM + am) * 8] = lv;
+    }
+  };
+  auto load_b = [&](half8_t (&bh)[8], half8_t (&bl)[8], int st) __attribute__((always_inline)) {
+    const int soff = ((k0 + st * WK) / 8) * ldb * 16;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bh[b] = __builtin_bit_cast(half8_t, __builtin_amdgcn_raw_buffer_load_b128(hrsrc, bvoff, soff + b * 256, 0));
+      bl[b] = __builtin_bit_cast(half8_t, __builtin_amdgcn_raw_buffer_load_b128(lrsrc, bvoff, soff + b * 256, 0));
+    }
+  };
+  auto read_a = [&](half8_t &ah, half8_t &al, const _Float16 *as, int a) __attribute__((always_inline)) {
+    const int row = a * 16 + li;
+    al = *(const half8_t *)&as[((4 + lg) * QM + row) * 8];
+    ah = *(const half8_t *)&as[(lg * QM + row) * 8];
+  };
+  auto barrier = [&]() __attribute__((always_inline)) {
+    // LDS stores done, then the workgroup barrier; the memory clobber keeps
+    // the compiler from moving LDS accesses across it
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+
+  float ra[16];
+  int lim;
+  half8_t bh[8], bl[8], nbh[8], nbl[8], ah[8], al[8];
+  // prologue: stages 0 and 1, step 0's twiddles, step 2's trajectory values
+  load_b(bh, bl, 0);
+  load_a(ra, lim, 0);
+  store_a(ra, lim, As);
+  if (nsteps > 1) {
+    load_a(ra, lim, 1);
+    store_a(ra, lim, As + P_AST);
+  }
+  load_a(ra, lim, min(2, nsteps - 1));
+  barrier();
+#pragma unroll
+  for (int a = 0; a < 8; a++) read_a(ah[a], al[a], As, a);
+  int cur = 0;
+#pragma unroll 1
+  for (int st = 0; st < nsteps; st++) {
+    const int n1 = cur == 2 ? 0 : cur + 1, n2 = n1 == 2 ? 0 : n1 + 1;
+    const bool more = st + 1 < nsteps;
+    load_b(nbh, nbl, min(st + 1, nsteps - 1));
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+      // this row block's products are issued: its registers take step
+      // st + 1's fragments (stage n1, complete since the last barrier)
+      if (more) read_a(ah[a], al[a], As + n1 * P_AST, a);
+    }
+    if (st + 2 < nsteps) store_a(ra, lim, As + n2 * P_AST);
+    load_a(ra, lim, min(st + 3, nsteps - 1));
+#pragma unroll
+    for (int b = 0; b < 8; b++) { bh[b] = nbh[b]; bl[b] = nbl[b]; }
+    barrier();
+    cur = n1;
+  }
+  // Y update: one row pointer per (a, r), the eight column blocks at
+  // constant offsets from it
+  float *yrow = Yb + (size_t)(m0 + lg * 4) * ldb + j0 + wj + li;
+  const bool first = k0 == 0;
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (m0 + a * 16 + lg * 4 + r < M) {
+        float *yp = yrow + (size_t)(a * 16 + r) * ldb;
+        if (first) {
+#pragma unroll
+          for (int b = 0; b < 8; b++) yp[b * 16] = acc[a][b][r];
+        } else {
+#pragma unroll
+          for (int b = 0; b < 8; b++) unsafeAtomicAdd(yp + b * 16, acc[a][b][r]);
+        }
+      }
+    }
+}
+
 // B8h / B8l: twiddles in the chunk-major layout [T' / 8][ldb][8] (T' a multiple of 32)
 void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const float *A,
                              const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
@@ -569,6 +745,20 @@ void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const flo
   const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
   dim3 grid(nm * (ldb / W_TJ), 1, S);
   hipLaunchKernelGGL(time_gemm_mfma_w_kernel, grid, dim3(W_THREADS), 0, st,
+                     (const GemmSample *)samples_dev, A, B8h, B8l, k0, K, ldb, Y);
+}
+
+// the pipelined 128 x 512 tile, same operands and preconditions as above
+void launch_time_gemm_mfma_p(const void *samples_dev, int S, int maxM, const float *A,
+                             const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
+                             float *Y, hipStream_t st) {
+  if (ldb % W_TJ != 0) throw Error(HSMD_ERR_ARG, "2 * n_omega must be a multiple of 512");
+  if (k0 % WK != 0) throw Error(HSMD_ERR_ARG, "trajectory chunks start at multiples of 32 points");
+  if ((long long)((k0 + K + WK - 1) / 8 + 4) * ldb * 16 >= (1LL << 31))
+    throw Error(HSMD_ERR_ARG, "the twiddle table exceeds 2 GiB");
+  const int nm = (((maxM + QM - 1) / QM) + 7) / 8 * 8;
+  dim3 grid(nm * (ldb / W_TJ), 1, S);
+  hipLaunchKernelGGL(time_gemm_mfma_p_kernel, grid, dim3(P_THREADS), 0, st,
                      (const GemmSample *)samples_dev, A, B8h, B8l, k0, K, ldb, Y);
 }
 

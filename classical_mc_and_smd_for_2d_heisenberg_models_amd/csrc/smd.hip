@@ -161,6 +161,9 @@ void launch_time_gemm_mfma(const void *samples_dev, int S, int maxM, const float
 void launch_time_gemm_mfma_w(const void *samples_dev, int S, int maxM, const float *A,
                              const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
                              float *Y, hipStream_t st);
+void launch_time_gemm_mfma_p(const void *samples_dev, int S, int maxM, const float *A,
+                             const _Float16 *B8h, const _Float16 *B8l, int k0, int K, int ldb,
+                             float *Y, hipStream_t st);
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
                                 int ldb, float *Y, hipStream_t st, int tj, bool db);
@@ -501,9 +504,16 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
     while (e->Tc > 256 && need(e->Tc) > 0.9 * (double)freeb) e->Tc = lower(e->Tc);
     if (need(e->Tc) > 0.9 * (double)freeb)
       throw Error(HSMD_ERR_DEVICE, "the DSSF buffers of this batch exceed the free device memory");
+    // the contraction addresses a sample's trajectory chunk with 32-bit byte
+    // offsets (buffer loads): the chunk shrinks until its Tc x ldm FP32 values
+    // stay below 2^31 B for every sample
+    if (e->f32) {
+      long long ldmax = 0;
+      for (int s = 0; s < S; s++)
+        ldmax = std::max(ldmax, ((3LL * e->lats[sample_lattice[s]].N + 3) / 4) * 4);
+      while (e->Tc > 256 && (long long)e->Tc * ldmax * 4 >= (1LL << 31)) e->Tc = lower(e->Tc);
+    }
   }
-  // the contraction addresses a sample's trajectory chunk with 32-bit byte
-  // offsets (buffer loads): its Tc x ldm FP32 values must stay below 2^31 B
   if (e->f32 && sp.dssf)
     for (int s = 0; s < S; s++) {
       const long long ldm = ((3LL * e->lats[sample_lattice[s]].N + 3) / 4) * 4;
@@ -717,7 +727,10 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.gemm == 4)
+  if (e.gemm == 5)
+    launch_time_gemm_mfma_p(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw8_hi.p,
+                            e.tw8_lo.p, e.chunk_base, e.chunk_fill, ldb, (float *)e.ybuf.p, 0);
+  else if (e.gemm == 4)
     launch_time_gemm_mfma_w(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw8_hi.p,
                             e.tw8_lo.p, e.chunk_base, e.chunk_fill, ldb, (float *)e.ybuf.p, 0);
   else if (e.f32 && e.ldt > 0 && e.gemm != 1)
@@ -1075,19 +1088,21 @@ static int current_form(const SMDEngine &e) {
 // sample, stages in HBM / Infinity Cache).  gemm: 0 = 128 x 128 contraction
 // tile with pre-split twiddles (FP32, 2 n_omega a multiple of 256), 1 = 64 x 64,
 // 2 = 128 x 256 with pre-split twiddles, 3 = 128 x 128 with two LDS stages,
-// 4 = 128 x 512 with chunk-major twiddles (2 n_omega a multiple of 512).
+// 4 = 128 x 512 with chunk-major twiddles (2 n_omega a multiple of 512),
+// 5 = the same tile pipelined (twiddles in registers, trajectory through a
+// three-stage LDS ring; same sums as 4).
 // Per-site arithmetic is the same in every shape; FP64 reductions may group
 // terms differently.
 void smd_set_shape(SMDEngine &e, int form, int gemm) {
   if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
   if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
-  if (gemm < 0 || gemm > 4) throw Error(HSMD_ERR_ARG, "gemm tile must be 0..4");
+  if (gemm < 0 || gemm > 5) throw Error(HSMD_ERR_ARG, "gemm tile must be 0..5");
   if ((form == 1 || form == 2) && !e.lds_ok)
     throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
   if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
   if (gemm != 1 && !(e.f32 && e.ldt > 0))
     throw Error(HSMD_ERR_ARG, "the 128-row MFMA tiles need FP32 and 2 n_omega a multiple of 256");
-  if (gemm == 4 && e.tw8_hi.n == 0)
+  if (gemm >= 4 && e.tw8_hi.n == 0)
     throw Error(HSMD_ERR_ARG, "the 128 x 512 MFMA tile needs FP32 and 2 n_omega a multiple of 512");
   if (form == 1 || form == 2) {
     e.use_lds = true;
@@ -1190,7 +1205,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
     if (sp.dssf && e->f32) {
       std::vector<int> tiles = {1};
       if (e->ldt > 0) tiles = {0, 3, 2, 1};
-      if (e->tw8_hi.n > 0) tiles.insert(tiles.begin(), 4);
+      if (e->tw8_hi.n > 0) tiles.insert(tiles.begin(), {5, 4});
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);

@@ -303,7 +303,7 @@ hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
 /* Result of hsmd_smd_autotune. */
 typedef struct hsmd_smd_tuning {
   int32_t form;              /* for hsmd_smd_set_shape: 1, 2 or 3 */
-  int32_t gemm;              /* for hsmd_smd_set_shape: 0..4 */
+  int32_t gemm;              /* for hsmd_smd_set_shape: 0..5 */
   double ms_per_step;        /* RK4 step of the whole batch, chosen form */
   double default_ms_per_step;/* same, form chosen at creation */
   double gemm_ms;            /* contraction of one trajectory chunk of up to 2048 points,
@@ -319,7 +319,10 @@ typedef struct hsmd_smd_tuning {
  * classes (FP32 batches of two classes), 3 = global-memory form.  gemm:
  * 0 = 128 x 128 DSSF contraction tile, 2 = 128 x 256, 3 = 128 x 128 with
  * two LDS stages (all three FP32 with 2 n_omega a multiple of 256),
- * 4 = 128 x 512 (FP32 with 2 n_omega a multiple of 512), 1 = 64 x 64.
+ * 4 = 128 x 512 (FP32 with 2 n_omega a multiple of 512), 5 = the 128 x 512
+ * tile pipelined: twiddles in registers, trajectory through a three-stage
+ * LDS ring (same preconditions and sums as 4),
+ * 1 = 64 x 64.
  * Errors when the shape does
  * not apply to the batch. */
 hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm);

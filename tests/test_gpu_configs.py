@@ -514,8 +514,8 @@ def _feed_blocks(smd, trajs, step=256):
         smd.feed(np.concatenate([tr[t0:t1] for tr in trajs], axis=1))
 
 
-@pytest.mark.parametrize("gemm,max_chunk", [(4, 2048), (4, None), (0, 2048), (2, 2048),
-                                            (3, 2048), (1, 2048)])
+@pytest.mark.parametrize("gemm,max_chunk", [(5, 2048), (5, None), (4, 2048), (4, None),
+                                            (0, 2048), (2, 2048), (3, 2048), (1, 2048)])
 def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm, max_chunk, monkeypatch):
     """The split-FP16 MFMA contraction's own error: the oracle's FP64
     trajectories (square L=48 with its 230 vacancies and kagome L=32, two
@@ -567,7 +567,34 @@ def test_config4_mfma_dssf_of_a_known_trajectory_against_fp64(gemm, max_chunk, m
                 assert err < 1e-9
 
 
-@pytest.mark.parametrize("gemm", [4, 0, 2, 3, 1])
+@pytest.mark.parametrize("max_chunk", [2048, None])
+def test_pipelined_contraction_equals_the_128x512_tile(max_chunk, monkeypatch):
+    """Contraction tile 5 (the pipelined 128 x 512 tile: twiddles in registers)
+    forms the same split products in the same order as tile 4, so the DSSF
+    of the same fed trajectory must be bit-identical: square L=48 with its
+    vacancies and kagome L=32, two samples each, 2381 points of unit
+    vectors, 512 bins, with the chunk capped at 2048 points (a chunk that
+    stores Y, then a 333-point tail that adds to it and ends inside a
+    32-point step) and uncapped (one 2381-point chunk)."""
+    if max_chunk:
+        monkeypatch.setenv("HSMD_SMD_MAX_CHUNK", str(max_chunk))
+    _say("pipelined contraction vs 128 x 512 tile, max chunk", max_chunk)
+    c, sub, vac, idx, samples = _config4_dssf_case()
+    T, no, dt = 2381, 512, 0.002
+    rng = np.random.default_rng(77)
+    trajs = [_unit(rng, (T, sub[idx[k]].N, 3)) for k in range(4)]
+    out = {}
+    for gemm in (4, 5):
+        smd = H.SpinDynamics(sub, idx, dt, T, n_omega=no, precision=H.BATCHED_F32)
+        smd.set_shape(0, gemm)
+        _feed_blocks(smd, trajs)
+        out[gemm] = [smd.dssf(li)[0] for li in range(2)]
+    for li in range(2):
+        assert np.isfinite(out[5][li]).all()
+        assert np.array_equal(out[4][li], out[5][li]), np.abs(out[4][li] - out[5][li]).max()
+
+
+@pytest.mark.parametrize("gemm", [5, 4, 0, 2, 3, 1])
 def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm, monkeypatch):
     """The same batch integrated by both engines: FP32 RK4 with the MFMA
     contraction against FP64 RK4 with the FP64 GEMM over 2381 time points

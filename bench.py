@@ -469,9 +469,11 @@ def bench_mc(timer, args, rng, tune):
     mc.sweeps_only(ksteps)
     kms = timer.stop() / ksteps
     thr = mc.tuning["threads"] if mc.tuning else 1024
-    mc_kernel = ("heis_sweep_fast<float,20,true>" if np.bincount(lat.colours()).max() <= thr
-                 else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
     z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
+    # the eighteen-slot float4 replica statically placed in LDS (the workload
+    # here) when every colour class fits one round of the workgroup
+    mc_kernel = ("heis_sweep_fast<float,18,true,false,true>" if np.bincount(lat.colours()).max() <= thr
+                 else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
     gather_bytes = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
     # HBM bytes a sweep must move with each replica resident in LDS: one read
     # and one write of its spins (12 B per site) plus one pass over the packed
@@ -484,7 +486,12 @@ def bench_mc(timer, args, rng, tune):
                              "bytes_per_update": gather_bytes,
                              "achieved": ups_per_sweep * gather_bytes / (kms * 1e-3) / 1e9,
                              "unit": "GB/s",
-                             "note": "neighbour gathers served from LDS, not HBM"}})
+                             "note": "neighbour gathers served from LDS, not HBM"},
+                             # LDS view: 20 float4 accesses per update (z neighbours,
+                             # the own spin and its store)
+                             "lds": {"bytes_per_launch": ups_per_sweep * 16 * (z + 2),
+                                     "achieved": ups_per_sweep * 16 * (z + 2) / (kms * 1e-3) / 1e9,
+                                     "unit": "GB/s"}})
     print("[bench] MC batched f32: %.3f ms/sweep, %.3e spin-updates/s, kernel %.3f ms, VALU frac %s"
           % (ms / args.steps, value, kms, roof["frac"]), file=sys.stderr, flush=True)
     out = {

@@ -80,10 +80,25 @@ __device__ __forceinline__ double field_energy_term(const LatDev &L, const T *sz
 // FP64 normalisation then shortens them, so E rose by 2 <|S| - 1> |E| every
 // step (1.7e-4 per step on 4096 sites).  Rounding an FP64 unit vector to
 // FP32 is unbiased.
+// 1 / sqrt(n2) in FP64.  FP32 spins: from the hardware's FP64 reciprocal
+// square-root estimate and one Newton step (relative error ~1e-13, far below
+// the FP32 rounding of the stored spin, so the rounding stays unbiased)
+// instead of a correctly rounded FP64 square root and division; FP64 spins
+// keep the correctly rounded form.
+template <typename T>
+__device__ __forceinline__ double inv_norm(double n2) {
+  if constexpr (sizeof(T) == 4) {
+    const double r = __builtin_amdgcn_rsq(n2);
+    return r * (1.5 - 0.5 * n2 * r * r);
+  } else {
+    return 1.0 / sqrt(n2);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void unit_store(const T (&v)[3], T &x, T &y, T &z) {
   const double a = v[0], b = v[1], c = v[2];
-  const double inr = 1.0 / sqrt(a * a + b * b + c * c);
+  const double inr = inv_norm<T>(a * a + b * b + c * c);
   x = (T)(a * inr);
   y = (T)(b * inr);
   z = (T)(c * inr);
@@ -460,8 +475,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           double x = (double)Sx[p] + cfac * D[u][0];
           double y = (double)Sy[p] + cfac * D[u][1];
           double z = (double)Sz[p] + cfac * D[u][2];
-          const double nr = sqrt(x * x + y * y + z * z);
-          const double inr = 1.0 / nr;
+          const double inr = inv_norm<T>(x * x + y * y + z * z);
           x *= inr; y *= inr; z *= inr;
           qs += spin0(u, p, 0) * x + spin0(u, p, 1) * y + spin0(u, p, 2) * z;
           const T tx = (T)x, ty = (T)y, tz = (T)z;
@@ -676,8 +690,7 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel_wide(const StepArgs a)
         double x = (double)Sx[p] + cfac * D[p];
         double y = (double)Sy[p] + cfac * D[Npad + p];
         double z = (double)Sz[p] + cfac * D[2 * Npad + p];
-        const double nr = sqrt(x * x + y * y + z * z);
-        const double inr = 1.0 / nr;
+        const double inr = inv_norm<T>(x * x + y * y + z * z);
         x *= inr; y *= inr; z *= inr;
         qs += (double)g0[p] * x + (double)g0[Npad + p] * y + (double)g0[2 * Npad + p] * z;
         const T tx = (T)x, ty = (T)y, tz = (T)z;

@@ -468,14 +468,14 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // sizes and layout
   const int T_ = sp.n_steps;
   // trajectory chunk: the first contraction of a run stores Y and every
-  // later one adds to it (a pass over all of Y, at 512 bins as large as a
-  // 1024-point chunk), and each contraction launch pays its tile epilogues
-  // once, so long chunks are cheaper: the FP32 chunk holds up to 8192 points
-  // (the 512-sample batch of 1.6M sites: 160 GB, one contraction per
-  // 8192-step run, 151 ms against 4 x 2048-point chunks taking 198 ms).  The
-  // chunk shrinks to the next lower power of two (down to 256 points) while
-  // it, Y and the spin buffers would not fit in 90 % of the free device
-  // memory; HSMD_SMD_MAX_CHUNK caps it (a power of two >= 256).
+  // later one adds to it (a read and a write of all of Y, at 512 bins as
+  // large as a 1024-point chunk), and each contraction launch pays its tile
+  // prologues and epilogues once, so the FP32 chunk holds up to 8192 points
+  // (the 512-sample batch of 1.6M sites: 160 GB, one contraction of 210 ms
+  // per 8192-step run).  The chunk shrinks to the next lower power of two
+  // (down to 256 points) while it, Y and the spin buffers would not fit in
+  // 90 % of the free device memory; HSMD_SMD_MAX_CHUNK caps it (a power of
+  // two >= 256).
   e->nop = e->f32 ? ((sp.n_omega + 127) / 128) * 128 : ((sp.n_omega + 31) / 32) * 32;
   const int no = e->nop;
   int cap = e->f32 ? 8192 : 256;
@@ -553,7 +553,7 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   // launch shape
   e->spt = 1;
   while (e->spt * kSmdThreads < maxN) e->spt *= 2;
-  e->lds_npad = e->spt * kSmdThreads + 1;   // the kernels' compile-time component stride
+  e->lds_npad = e->spt * kSmdThreads + 64;   // the kernels' compile-time component stride
   e->lds_bytes = (size_t)3 * 3 * e->lds_npad * es;   // S, B1, B2
   e->use_lds = (sp.precision != HSMD_PLAIN_F64) && e->lds_bytes <= 150 * 1024 &&
                e->spt * (int)es <= 16;

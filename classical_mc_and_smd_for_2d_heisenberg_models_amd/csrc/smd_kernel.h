@@ -301,7 +301,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
   if (LDS) {
     // a compile-time component stride: the y and z reads of a neighbour then
     // take the x address plus an immediate LDS offset
-    stride = SPT * kSmdThreads + 1;
+    stride = SPT * kSmdThreads + 64;
     S = (T *)smem;
     B1 = S + 3 * stride;
     B2 = B1 + 3 * stride;

@@ -472,13 +472,14 @@ def bench_mc(timer, args, rng, tune):
     z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
     # the eighteen-slot float4 replica statically placed in LDS (the workload
     # here) when every colour class fits one round of the workgroup
-    mc_kernel = ("heis_sweep_fast<float,18,true,false,true>" if np.bincount(lat.colours()).max() <= thr
+    mc_kernel = ("heis_sweep_fast<float,18,true,false,true,5>" if np.bincount(lat.colours()).max() <= thr
                  else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
     gather_bytes = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
     # HBM bytes a sweep must move with each replica resident in LDS: one read
     # and one write of its spins (12 B per site) plus one pass over the packed
-    # 16-bit neighbour labels ((z + 1) / 2 words per site, shared by all replicas)
-    hbm_bytes = R * 2 * 12 * (N + 1) + 4 * ((z + 1) // 2) * N
+    # 16-bit neighbour labels ((z + 1) / 2 words per site, shared by all
+    # replicas) and over the energy's half-shell labels ((z / 2 + 1) / 2 words)
+    hbm_bytes = R * 2 * 12 * (N + 1) + 4 * ((z + 1) // 2 + (z // 2 + 1) // 2) * N
     att, acc = mc.pt_counters()
     roof = valu_roofline("mc_sweep_f32_triang96_r256_valu_pmc.txt", kms * 1e-3, hbm_bytes,
                          mc_kernel, traffic_profile="mc_sweep_f32_triang96_r256_pmc.txt",

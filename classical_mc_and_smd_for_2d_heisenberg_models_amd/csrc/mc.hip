@@ -84,6 +84,12 @@ struct SweepArgs {
   int table_energy;
   uint32_t emask;
   double escale;
+  // the same energy from a label table of the emask slots alone: nEw words
+  // per site of packed 16-bit pairs (sentinel-padded), couplings ejxy / ejz
+  // per emask slot in slot order (zero for the padding)
+  const uint32_t *nbE;
+  int nEw;
+  double ejxy[16], ejz[16];
   // compact labels (sparse tables, fast kernels): each site's coupled
   // neighbours first, in slot order, as 14-bit labels with the 2-bit shell
   // (0..2) of their slot; wcnt[col * 16 + chunk] = label words that hold
@@ -553,7 +559,10 @@ __device__ __forceinline__ uint32_t lab_hi16(uint32_t w) {
 // (odd) class; a site of an odd class whose thread had none in the class
 // before draws its own block.  Counters stay (site, sweep * 16 + pass,
 // configuration, 0), so no two draws share one.
-template <typename T, int ZC, bool AOS, bool CP = false, bool STAT = false>
+// EW > 0 (with the table energy): the energy from the emask-slot label
+// table of EW words per site (nbE), every neighbour read issued before the
+// products, instead of the full table with a branch per slot.
+template <typename T, int ZC, bool AOS, bool CP = false, bool STAT = false, int EW = 0>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void heis_sweep_fast(const SweepArgs a) {
   static_assert(!AOS || sizeof(T) == 4, "the float4 layout is FP32 only");
@@ -780,36 +789,105 @@ void heis_sweep_fast(const SweepArgs a) {
     iteration(it, qa, la, qb, lb);
     if (it + 1 < total) iteration(it + 1, qb, lb, qa, la);
   }
-  if (a.table_energy) {
+  bool written = false;
+  if constexpr (EW > 0) {
+    // E = escale sum_i S_i . sum_{e < 2 EW} (ej_e S_nb(i, e)) - h sum_i z_i over
+    // the emask slots in slot order: the same FP64 products and sums as the
+    // branch-per-slot form below (a padding slot adds J 0 times the
+    // sentinel's zero spin); EB sites' labels per batch, all 2 EW
+    // neighbour reads of a site ahead of its products.  Each site's spin goes
+    // back to global memory here, where it is in registers anyway.
+    constexpr int EB = 3;
+    const __amdgpu_buffer_rsrc_t ersrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.nbE, (short)0, 0x7fffffff, 0x00020000);
+    double e = 0.0, mz = 0.0;
+    for (int p0 = tid; p0 < N; p0 += EB * blockDim.x) {
+      uint32_t q[EB][EW];
+#pragma unroll
+      for (int b = 0; b < EB; b++) {
+        const int pc = 4 * min(p0 + b * (int)blockDim.x, N - 1);
+#pragma unroll
+        for (int k = 0; k < EW; k++) q[b][k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ersrc, pc, 4 * k * N, 0);
+      }
+#pragma unroll
+      for (int b = 0; b < EB; b++) {
+        const int p = p0 + b * (int)blockDim.x;
+        if (p < N) {
+          T u[2 * EW], v[2 * EW], w[2 * EW];
+#pragma unroll
+          for (int k = 0; k < EW; k++) {
+            nget(q[b][k], 0, u[2 * k], v[2 * k], w[2 * k]);
+            nget(q[b][k], 1, u[2 * k + 1], v[2 * k + 1], w[2 * k + 1]);
+          }
+          T x, y, z;
+          get(p, x, y, z);
+          double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+          for (int k = 0; k < 2 * EW; k++) {
+            ax = fma(a.ejxy[k], (double)u[k], ax);
+            ay = fma(a.ejxy[k], (double)v[k], ay);
+            az = fma(a.ejz[k], (double)w[k], az);
+          }
+          e = fma((double)x, ax, e);
+          e = fma((double)y, ay, e);
+          e = fma((double)z, az, e);
+          mz += (double)z;
+          g[p] = x; g[Npad + p] = y; g[2 * Npad + p] = z;
+        }
+      }
+    }
+    written = true;   // the sentinel site N keeps its zero spin
+    e *= a.escale;
+    if (a.cp.h_on) e -= a.cp.h * mz;
+    double v1[1] = {e};
+    block_sum<1>(v1, scratch);
+    if (threadIdx.x == 0) a.energy[c] = v1[0];
+  } else if (a.table_energy) {
     // E = escale sum_i S_i . sum_{s in emask} (J_s S_nb(i,s)) - h sum_i z_i,
     // every product and sum in FP64; the labels are read again (coalesced,
-    // from L2) and the spins from LDS
+    // from L2), EB sites' worth per batch so their latency is met once per
+    // batch (one site in FP64, whose registers a batch would spill), and the
+    // spins from LDS.  Each site's spin goes back to global memory here,
+    // where it is in registers anyway.
+    constexpr int EB = sizeof(T) == 4 ? 3 : 1;
     double e = 0.0, mz = 0.0;
-    for (int p = tid; p < N; p += blockDim.x) {
-      uint32_t q[ZH];
+    for (int p0 = tid; p0 < N; p0 += EB * blockDim.x) {
+      uint32_t q[EB][ZH];
 #pragma unroll
-      for (int k = 0; k < ZH; k++) q[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, 4 * p, 4 * k * N, 0);
-      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int b = 0; b < EB; b++) {
+        const int pc = 4 * min(p0 + b * (int)blockDim.x, N - 1);
 #pragma unroll
-      for (int k = 0; k < ZH; k++)
+        for (int k = 0; k < ZH; k++) q[b][k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, pc, 4 * k * N, 0);
+      }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const int sl = 2 * k + h;
-          if (a.emask & (1u << sl)) {
-            T u, v, w;
-            nget(q[k], h, u, v, w);
-            ax = fma(a.cp.jxy[sl], (double)u, ax);
-            ay = fma(a.cp.jxy[sl], (double)v, ay);
-            az = fma(a.cp.jz[sl], (double)w, az);
-          }
+      for (int b = 0; b < EB; b++) {
+        const int p = p0 + b * (int)blockDim.x;
+        if (p < N) {
+          double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+          for (int k = 0; k < ZH; k++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+              const int sl = 2 * k + h;
+              if (a.emask & (1u << sl)) {
+                T u, v, w;
+                nget(q[b][k], h, u, v, w);
+                ax = fma(a.cp.jxy[sl], (double)u, ax);
+                ay = fma(a.cp.jxy[sl], (double)v, ay);
+                az = fma(a.cp.jz[sl], (double)w, az);
+              }
+            }
+          T x, y, z;
+          get(p, x, y, z);
+          e = fma((double)x, ax, e);
+          e = fma((double)y, ay, e);
+          e = fma((double)z, az, e);
+          mz += (double)z;
+          g[p] = x; g[Npad + p] = y; g[2 * Npad + p] = z;
         }
-      T x, y, z;
-      get(p, x, y, z);
-      e = fma((double)x, ax, e);
-      e = fma((double)y, ay, e);
-      e = fma((double)z, az, e);
-      mz += (double)z;
+      }
     }
+    written = true;   // the sentinel site N keeps its zero spin
     e *= a.escale;
     if (a.cp.h_on) e -= a.cp.h * mz;
     double v1[1] = {e};
@@ -822,6 +900,7 @@ void heis_sweep_fast(const SweepArgs a) {
       x = (double)u; y = (double)v; z = (double)w;
     });
   }
+  if (written) return;
   if constexpr (AOS) {
     for (int i = threadIdx.x; i < Npad; i += blockDim.x) {
       const float4 v = s4[i];
@@ -1121,6 +1200,7 @@ struct MCEngine {
   std::vector<double> temps;
   DevBuf<int32_t> nb, perm, bi, bj, z2nb;
   DevBuf<uint32_t> nb16;
+  DevBuf<uint32_t> nbE;   // emask-slot labels of the table energy
   bool fast = false;
   // launch shape of the batched sweeps: threads per replica workgroup; the
   // register-label kernel needs every colour class to fit one pass of them
@@ -1483,6 +1563,31 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
       e->nb16.upload(pk.data(), pk.size());
       a.nb16 = e->nb16.p;
       if (!e->compact) pick_energy_form(lat, a.cp, a);
+      a.nbE = nullptr;
+      a.nEw = 0;
+      for (int k = 0; k < 16; k++) a.ejxy[k] = a.ejz[k] = 0.0;
+      if (a.table_energy && __builtin_popcount(a.emask) <= 16) {
+        // the emask slots' labels in slot order, two per word, sentinel-padded
+        std::vector<int> sl;
+        for (int s = 0; s < lat.Z; s++)
+          if (a.emask & (1u << s)) sl.push_back(s);
+        const int nw = ((int)sl.size() + 1) / 2;
+        std::vector<uint32_t> pe((size_t)nw * N, (uint32_t)N | ((uint32_t)N << 16));
+        for (int k = 0; k < nw; k++)
+          for (int p = 0; p < N; p++) {
+            const uint32_t lo = (uint32_t)lat.g_nb[(size_t)sl[2 * k] * N + p];
+            const uint32_t hi = 2 * k + 1 < (int)sl.size() ? (uint32_t)lat.g_nb[(size_t)sl[2 * k + 1] * N + p]
+                                                            : (uint32_t)N;
+            pe[(size_t)k * N + p] = lo | (hi << 16);
+          }
+        for (size_t k = 0; k < sl.size(); k++) {
+          a.ejxy[k] = a.cp.jxy[sl[k]];
+          a.ejz[k] = a.cp.jz[sl[k]];
+        }
+        e->nbE.upload(pe.data(), pe.size());
+        a.nbE = e->nbE.p;
+        a.nEw = nw;
+      }
     }
   }
   if (e->use_lds) HSMD_ZC_DISPATCH(lat.Z, set_lds_attr, *e);
@@ -1629,8 +1734,12 @@ static void launch_sweep(MCEngine &e, hipStream_t st) {
     // eighteen coupled slots (triangular lattice with three shells): the
     // statically placed kernel with exactly nine label words per site
     e.args.config_base = 0;
-    hipLaunchKernelGGL((heis_sweep_fast<float, 18, true, false, true>), dim3(e.R), dim3(e.threads), 0,
-                       st, e.args);
+    if (a.table_energy && a.nEw == 5)   // nine emask slots: the half-shell table energy
+      hipLaunchKernelGGL((heis_sweep_fast<float, 18, true, false, true, 5>), dim3(e.R), dim3(e.threads), 0,
+                         st, e.args);
+    else
+      hipLaunchKernelGGL((heis_sweep_fast<float, 18, true, false, true>), dim3(e.R), dim3(e.threads), 0,
+                         st, e.args);
     return;
   }
   HSMD_ZC_DISPATCH(e.lat.Z, launch_sweep_zc, e, st);

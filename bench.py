@@ -58,7 +58,8 @@ SMD_CHUNK = 8192
 # DSSF time-contraction kernel per contraction tile
 GEMM_KERNEL = {0: "time_gemm_mfma_q_kernel", 1: "time_gemm_mfma_kernel",
                2: "time_gemm_mfma_q256_kernel", 3: "time_gemm_mfma_db_kernel",
-               4: "time_gemm_mfma_w_kernel", 5: "time_gemm_mfma_p_kernel"}
+               4: "time_gemm_mfma_w_kernel", 5: "time_gemm_mfma_p_kernel",
+               6: "time_dft_fft_kernel"}
 # RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
 SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
                    3: "smd_kernel_wide<float,12,false>"}
@@ -641,7 +642,7 @@ def bench_smd(timer, args, rng, tune):
     # FP32 trajectory row the contraction reads back (3 N values per sample)
     rk_hbm = sum(4 * 3 * lats[i].N for i in idx)
     form = smd.tuning["form"] if smd.tuning else 2
-    gemm = smd.tuning["gemm"] if smd.tuning else 4
+    gemm = smd.tuning["gemm"] if smd.tuning else 6
     roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
                          SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % nb,
                          unit="step",
@@ -650,28 +651,41 @@ def bench_smd(timer, args, rng, tune):
                                                  "achieved": rk_gather / (rk_ms * 1e-3) / 1e9,
                                                  "unit": "GB/s",
                                                  "note": "neighbour gathers served from LDS"}})
+    # the chunk's time transform: the trajectory chunk read once, Y (3 N rows x
+    # 2 n_omega FP32 per sample) written once
+    tr_bytes = 4 * gm * chunk + 4 * gm * 1024
+    if gemm == 6:
+        # the pruned FFT: VALU-issue bound (the profile's VALU instructions
+        # per launch over the live launch time), HBM beside it
+        troof = valu_roofline("dssf_fft_config4_pmc.txt", gemm_ms * 1e-3, tr_bytes,
+                              GEMM_KERNEL[6] + " (one %d-point chunk)" % chunk,
+                              extra={"algorithmic_flop_per_launch": alg_flop,
+                                     "algorithmic_tflops": alg_flop / (gemm_ms * 1e-3) / 1e12,
+                                     "note": "T / 512 decimated 512-point FFTs per row pair "
+                                             "and one complex multiply-add per bin and t2"})
+    else:
+        troof = {"bound": "mfma",
+                 "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
+                 "peak": 2500.0, "unit": "TFLOP/s",
+                 "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
+                 "traffic": read_traffic("dssf_contraction_mfma_pmc.txt"),
+                 "kernel": GEMM_KERNEL.get(gemm, "?"),
+                 "algorithmic_flop_per_launch": alg_flop,
+                 "executed_flop_per_launch": mfma_flop,
+                 "executed_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,
+                 "mfma_busy_frac": profile_value("dssf_contraction_mfma_pmc.txt", "mfma_busy_frac"),
+                 "profile": "profiles/dssf_contraction_mfma_pmc.txt"}
     out = {"value": srate, "unit": "site-RK4-steps/s", "steps": k,
            "ms_per_step": sms / k, "samples": len(idx), "sites": sites, "n_steps_total": T,
            "workload": "512 samples: square L=64, kagome L=32, hexagonal L=40, "
                        "square L=48 + 10% vacancies; FP32 RK4 dt=0.005, FP64 "
-                       "correction, DSSF time contraction over 512 bins included",
+                       "correction, DSSF time transform over 512 bins included",
            "tuning": smd.tuning, "roofline": roof,
-           "dssf_chunk_gemm": {"ms": gemm_ms, "time_points": chunk,
-                               "roofline": {"bound": "mfma",
-                                            "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,
-                                            "peak": 2500.0, "unit": "TFLOP/s",
-                                            "frac": alg_flop / (gemm_ms * 1e-3) / 2.5e15,
-                                            "traffic": read_traffic("dssf_contraction_mfma_pmc.txt"),
-                                            "kernel": GEMM_KERNEL.get(gemm, "?"),
-                                            "algorithmic_flop_per_launch": alg_flop,
-                                            "executed_flop_per_launch": mfma_flop,
-                                            "executed_tflops": mfma_flop / (gemm_ms * 1e-3) / 1e12,
-                                            "mfma_busy_frac": profile_value(
-                                                "dssf_contraction_mfma_pmc.txt", "mfma_busy_frac"),
-                                            "profile": "profiles/dssf_contraction_mfma_pmc.txt"}}}
+           "dssf_chunk_transform": {"ms": gemm_ms, "time_points": chunk, "form": gemm,
+                                    "roofline": troof}}
     print("[bench] SMD batched f32: %.4f ms/step, %.3e site-steps/s, RK4 kernel %.4f ms/step, "
-          "chunk GEMM %.2f ms (%.0f TFLOP/s executed)"
-          % (sms / k, srate, rk_ms, gemm_ms, mfma_flop / (gemm_ms * 1e-3) / 1e12),
+          "chunk transform %.2f ms (%s)"
+          % (sms / k, srate, rk_ms, gemm_ms, GEMM_KERNEL.get(gemm, "?")),
           file=sys.stderr, flush=True)
     del smd
     # the same batch with FP64 state (global-memory form, FP64 GEMM contraction)
@@ -695,33 +709,37 @@ def bench_smd(timer, args, rng, tune):
 def dssf_error(rng, lats, samples, idx):
     """Accuracy of the FP32 form's DSSF against the FP64 form on a subset of
     the configuration-4 batch (two samples each of square L=48 with its
-    vacancies and kagome L=32), 2381 time points (a full 2048-point chunk and
-    a ragged tail), dt = 0.002, 512 bins: the integrated runs (FP32 RK4 +
-    split-FP16 MFMA contraction against FP64 RK4 + FP64 GEMM), and the
-    contraction alone on the FP64 run's own trajectory fed to both engines."""
+    vacancies and kagome L=32), 2560 time points (one chunk), dt = 0.002,
+    512 bins: the integrated runs, FP32 RK4 with the pruned-FFT time
+    transform (the configuration-4 default) and with the split-FP16 MFMA
+    contraction (128 x 512 tile), each against FP64 RK4 + FP64 GEMM."""
     sub = [lats[3], lats[1]]
     pick = [int(np.nonzero(idx == 3)[0][0]), int(np.nonzero(idx == 1)[0][0]),
             int(np.nonzero(idx == 3)[0][1]), int(np.nonzero(idx == 1)[0][1])]
     sidx = np.array([0, 1, 0, 1], np.int32)
     ss = [samples[k] for k in pick]
-    T, no, dt = 2381, 512, 0.002
+    T, no, dt = 2560, 512, 0.002
     res = {}
-    for prec in (H.BATCHED_F64, H.BATCHED_F32):
+    for prec, gemm in ((H.BATCHED_F64, None), (H.BATCHED_F32, 6), (H.BATCHED_F32, 4)):
         e = H.SpinDynamics(sub, sidx, dt, T, n_omega=no, precision=prec)
+        if gemm is not None:
+            e.set_shape(0, gemm)
         e.set_samples(ss)
         e.step(T - 1)
-        res[prec] = [e.dssf(li)[0] for li in range(2)]
-    out = {"subset": "2 x square L=48 (230 vacancies) + 2 x kagome L=32, 2381 points, dt=0.002, "
+        res[gemm] = [e.dssf(li)[0] for li in range(2)]
+    out = {"subset": "2 x square L=48 (230 vacancies) + 2 x kagome L=32, 2560 points, dt=0.002, "
                      "512 bins", "floor": "bins above 1e-2 of their (q, component) peak"}
-    pk, rl = [], []
-    for li in range(2):
-        a, b = res[H.BATCHED_F64][li], res[H.BATCHED_F32][li]
-        pk.append(np.abs(a - b).max() / a.max())
-        big = a > 1e-2 * a.max(axis=2, keepdims=True)
-        rl.append((np.abs(a - b)[big] / a[big]).max())
-    out["integrated"] = {"max_err_over_peak": float(max(pk)), "max_rel_err_above_floor": float(max(rl))}
-    print("[bench] DSSF FP32 vs FP64: peak-normalised %.2e, relative above floor %.2e"
-          % (max(pk), max(rl)), file=sys.stderr, flush=True)
+    for gemm, key in ((6, "integrated"), (4, "integrated_mfma_tile")):
+        pk, rl = [], []
+        for li in range(2):
+            a, b = res[None][li], res[gemm][li]
+            pk.append(np.abs(a - b).max() / a.max())
+            big = a > 1e-2 * a.max(axis=2, keepdims=True)
+            rl.append((np.abs(a - b)[big] / a[big]).max())
+        out[key] = {"form": GEMM_KERNEL[gemm], "max_err_over_peak": float(max(pk)),
+                    "max_rel_err_above_floor": float(max(rl))}
+        print("[bench] DSSF FP32 (%s) vs FP64: peak-normalised %.2e, relative above floor %.2e"
+              % (GEMM_KERNEL[gemm], max(pk), max(rl)), file=sys.stderr, flush=True)
     return out
 
 

@@ -476,8 +476,9 @@ class SpinDynamics:
         """Launch shape of the batched forms: form 1 (LDS, one slot class),
         2 (LDS, per-workgroup classes), 3 (global memory); gemm 0 (128 x 128
         contraction tile), 1 (64 x 64), 2 (128 x 256), 3 (128 x 128, two
-        LDS stages), 4 (128 x 512, 2 n_omega a multiple of 512) or 5 (the
-        128 x 512 tile pipelined, same sums as 4)."""
+        LDS stages), 4 (128 x 512, 2 n_omega a multiple of 512), 5 (the
+        128 x 512 tile pipelined, same sums as 4) or 6 (the time DFT as a
+        pruned FFT: FP32, n_steps a multiple of 512, n_omega <= 512)."""
         _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
 
     def set_samples_from_mc(self, mc, slots):

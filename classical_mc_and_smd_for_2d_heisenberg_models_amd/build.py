@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhsmd.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["lattice.cpp", "points.cpp", "api.cpp", "mc.hip", "sq.hip", "smd.hip", "smd_f32.hip", "smd_f64.hip", "dssf_mfma.hip", "dssf_direct.hip", "driver.cpp", "postproc.cpp", "tune.cpp", "qc.cpp", "stats.cpp", "noalloc.hip"]
+SOURCES = ["lattice.cpp", "points.cpp", "api.cpp", "mc.hip", "sq.hip", "smd.hip", "smd_f32.hip", "smd_f64.hip", "dssf_mfma.hip", "dssf_fft.hip", "dssf_direct.hip", "driver.cpp", "postproc.cpp", "tune.cpp", "qc.cpp", "stats.cpp", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -167,6 +167,11 @@ void launch_time_gemm_mfma_p(const void *samples_dev, int S, int maxM, const flo
 void launch_time_gemm_mfma_wide(const void *samples_dev, int S, int maxM, const float *A,
                                 const _Float16 *Bth, const _Float16 *Btl, int ldt, int k0, int K,
                                 int ldb, float *Y, hipStream_t st, int tj, bool db);
+std::vector<float> time_dft_fft_tables(int T);
+bool time_dft_fft_ok(int T, int nop);
+void launch_time_dft_fft(const void *samples_dev, int S, int maxM, const float *A, const float *win,
+                         const float *tables, int T, int k0, int K, int nop, int nkeep, float *Y,
+                         hipStream_t st);
 
 struct GemmSampleHost {
   int M;
@@ -213,6 +218,7 @@ struct SMDEngine {
   DevBuf<GemmSampleHost> gemm_samples;
   DevBuf<_Float16> tw_hi, tw_lo;   // twiddles split into FP16 hi/lo, [2 nop][ldt]
   DevBuf<_Float16> tw8_hi, tw8_lo; // the same, chunk-major [ldt / 8][2 nop][8] (2 nop % 512 == 0)
+  DevBuf<float> fft_tab, win32;    // FFT form: its twiddle tables and the FP32 window
   int nop = 0;                     // frequency bins held internally (n_omega padded)
   int ldt = 0;
   bool set = false;
@@ -223,7 +229,8 @@ struct SMDEngine {
   // launch shape (hsmd_smd_set_shape): LDS-resident form possible, one slot
   // class for every workgroup, and the contraction tile (0: 128 x 128 with
   // pre-split twiddles, 1: 64 x 64, 2: 128 x 256, 3: 128 x 128 with two
-  // LDS stages, 4: 128 x 512 with chunk-major twiddles)
+  // LDS stages, 4: 128 x 512 with chunk-major twiddles, 5: 4 pipelined,
+  // 6: the pruned FFT)
   bool lds_ok = false;
   bool one_class = false;
   int gemm = 0;
@@ -273,6 +280,13 @@ static void upload_twiddles(SMDEngine &e) {
       e.tw8_hi.upload(h8.data(), h8.size());
       e.tw8_lo.upload(l8.data(), l8.size());
     }
+  }
+  if (sizeof(T) == 4 && time_dft_fft_ok(T_, no)) {
+    const std::vector<float> tab = time_dft_fft_tables(T_);
+    e.fft_tab.upload(tab.data(), tab.size());
+    std::vector<float> w32(T_);
+    for (int t = 0; t < T_; t++) w32[t] = (float)e.window[t];
+    e.win32.upload(w32.data(), w32.size());
   }
 }
 
@@ -559,6 +573,10 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
                e->spt * (int)es <= 16;
   e->lds_ok = e->use_lds;
   e->gemm = e->tw8_hi.n > 0 ? 4 : (e->f32 && e->ldt > 0) ? 0 : 1;
+  // the FFT form costs one whole transform per chunk: the default while the
+  // run needs at most two chunks (a third of the 128 x 512 tile's time for
+  // one 8192-point chunk of the configuration-4 batch)
+  if (e->fft_tab.n > 0 && (T_ + e->Tc - 1) / e->Tc <= 2) e->gemm = 6;
   // register-resident global form only for the plain form's small lattices
   e->wide = !e->use_lds && (sp.precision != HSMD_PLAIN_F64 || e->spt > (e->f32 ? 8 : 4));
   if (!e->use_lds) {
@@ -727,7 +745,11 @@ static void flush_chunk(SMDEngine &e) {
   const int ldb = 2 * no;
   dim3 grid((maxM + 63) / 64, (ldb + 63) / 64, e.S);
   if (ldb % 64 != 0) throw Error(HSMD_ERR_ARG, "n_omega must be a multiple of 32");
-  if (e.gemm == 5)
+  if (e.gemm == 6)
+    launch_time_dft_fft(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.win32.p, e.fft_tab.p,
+                        e.spec.n_steps, e.chunk_base, e.chunk_fill, no, e.spec.n_omega,
+                        (float *)e.ybuf.p, 0);
+  else if (e.gemm == 5)
     launch_time_gemm_mfma_p(e.gemm_samples.p, e.S, maxM, (const float *)e.traj.p, e.tw8_hi.p,
                             e.tw8_lo.p, e.chunk_base, e.chunk_fill, ldb, (float *)e.ybuf.p, 0);
   else if (e.gemm == 4)
@@ -1090,19 +1112,23 @@ static int current_form(const SMDEngine &e) {
 // 2 = 128 x 256 with pre-split twiddles, 3 = 128 x 128 with two LDS stages,
 // 4 = 128 x 512 with chunk-major twiddles (2 n_omega a multiple of 512),
 // 5 = the same tile pipelined (twiddles in registers, trajectory through a
-// three-stage LDS ring; same sums as 4).
+// three-stage LDS ring; same sums as 4), 6 = the pruned FFT (FP32, n_steps a
+// multiple of 512, n_omega <= 512; its cost per chunk is that of a whole
+// run's transform once the chunk holds n_steps / 512 points).
 // Per-site arithmetic is the same in every shape; FP64 reductions may group
 // terms differently.
 void smd_set_shape(SMDEngine &e, int form, int gemm) {
   if (e.spec.precision == HSMD_PLAIN_F64) throw Error(HSMD_ERR_ARG, "the plain form has one launch shape");
   if (form < 0 || form > 3) throw Error(HSMD_ERR_ARG, "form must be 0..3");
-  if (gemm < 0 || gemm > 5) throw Error(HSMD_ERR_ARG, "gemm tile must be 0..5");
+  if (gemm < 0 || gemm > 6) throw Error(HSMD_ERR_ARG, "gemm tile must be 0..6");
   if ((form == 1 || form == 2) && !e.lds_ok)
     throw Error(HSMD_ERR_ARG, "the LDS-resident form does not fit these lattices");
   if (form == 2 && !two_class(e)) throw Error(HSMD_ERR_ARG, "the batch has a single slot class");
-  if (gemm != 1 && !(e.f32 && e.ldt > 0))
+  if (gemm == 6 && e.fft_tab.n == 0)
+    throw Error(HSMD_ERR_ARG, "the FFT form needs FP32, n_steps a multiple of 512 and n_omega <= 512");
+  if (gemm != 1 && gemm != 6 && !(e.f32 && e.ldt > 0))
     throw Error(HSMD_ERR_ARG, "the 128-row MFMA tiles need FP32 and 2 n_omega a multiple of 256");
-  if (gemm >= 4 && e.tw8_hi.n == 0)
+  if ((gemm == 4 || gemm == 5) && e.tw8_hi.n == 0)
     throw Error(HSMD_ERR_ARG, "the 128 x 512 MFMA tile needs FP32 and 2 n_omega a multiple of 512");
   if (form == 1 || form == 2) {
     e.use_lds = true;
@@ -1206,6 +1232,7 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
       std::vector<int> tiles = {1};
       if (e->ldt > 0) tiles = {0, 3, 2, 1};
       if (e->tw8_hi.n > 0) tiles.insert(tiles.begin(), {5, 4});
+      if (e->fft_tab.n > 0) tiles.insert(tiles.begin(), 6);
       gbest = 1e300;
       for (int g : tiles) {
         smd_set_shape(*e, best_form, g);
@@ -1217,7 +1244,15 @@ void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_latt
         flush_chunk(*e);   // warm-up
         e->chunk_base = 0;
         e->chunk_fill = kt;
-        const double ms = timed([&] { flush_chunk(*e); });
+        double ms = timed([&] { flush_chunk(*e); });
+        if (g == 6) {
+          // the FFT form costs one whole transform per chunk (of at least
+          // n_steps / 512 points): a run of n_steps points in chunks of up to
+          // 8192, scaled to the kt points the other tiles are timed on
+          const int P = sp.n_steps / 512;
+          const double full = ms * (double)P / std::min(P, kt);
+          ms = full * ((sp.n_steps + 8191) / 8192) * kt / sp.n_steps;
+        }
         if (getenv("HSMD_TUNE_VERBOSE")) fprintf(stderr, "[hsmd-tune] smd contraction tile %d: %.3f ms\n", g, ms);
         ncand++;
         if (g == def_gemm) gdef = ms;

@@ -303,11 +303,13 @@ hsmd_status hsmd_smd_create(const hsmd_lattice *const *lattices, int n_lattices,
 /* Result of hsmd_smd_autotune. */
 typedef struct hsmd_smd_tuning {
   int32_t form;              /* for hsmd_smd_set_shape: 1, 2 or 3 */
-  int32_t gemm;              /* for hsmd_smd_set_shape: 0..5 */
+  int32_t gemm;              /* for hsmd_smd_set_shape: 0..6 */
   double ms_per_step;        /* RK4 step of the whole batch, chosen form */
   double default_ms_per_step;/* same, form chosen at creation */
   double gemm_ms;            /* contraction of one trajectory chunk of up to 2048 points,
-                                chosen tile (-1: no DSSF) */
+                                chosen tile (-1: no DSSF); for the FFT form (6), whose
+                                chunk costs a whole transform, its cost over n_steps
+                                points in chunks of up to 8192, scaled to the chunk */
   double default_gemm_ms;
   int32_t n_candidates;      /* shapes timed; 0 when cached */
   int32_t cached;            /* 1: read from the tuning cache (HSMD_TUNE_CACHE) */
@@ -321,7 +323,8 @@ typedef struct hsmd_smd_tuning {
  * two LDS stages (all three FP32 with 2 n_omega a multiple of 256),
  * 4 = 128 x 512 (FP32 with 2 n_omega a multiple of 512), 5 = the 128 x 512
  * tile pipelined: twiddles in registers, trajectory through a three-stage
- * LDS ring (same preconditions and sums as 4),
+ * LDS ring (same preconditions and sums as 4), 6 = the time DFT as a pruned
+ * FFT (FP32 with n_steps a multiple of 512 and n_omega <= 512),
  * 1 = 64 x 64.
  * Errors when the shape does
  * not apply to the batch. */

@@ -594,6 +594,53 @@ def test_pipelined_contraction_equals_the_128x512_tile(max_chunk, monkeypatch):
         assert np.array_equal(out[4][li], out[5][li]), np.abs(out[4][li] - out[5][li]).max()
 
 
+@pytest.mark.parametrize("no,max_chunk", [(512, None), (512, 1024), (200, 1024), (200, None)])
+def test_fft_form_of_a_known_trajectory_against_fp64(no, max_chunk, monkeypatch):
+    """Contraction form 6, the time DFT as a pruned FFT (T / 512 decimated
+    512-point FFTs per row pair, combined per bin): square L=48 with its 230
+    vacancies, kagome L=32 and square L=10 (300 rows: a ragged last block of
+    32 rows), 2560 time points of unit vectors under a Gaussian window, fed
+    to the FP32 engine, against the FP64 DSSF of the same trajectories (FFT
+    form of the oracle's transform).  Uncapped the run is one chunk; capped
+    at 1024 points it is two chunks that store and add and a 512-point tail,
+    each decimated row cut by the chunk bounds.  At 200 bins the table holds
+    256 (bins 200..255 zero).  Bars: 1e-5 of the peak, 1e-4 relative on
+    every bin above 1e-2 of its (q, component) peak."""
+    if max_chunk:
+        monkeypatch.setenv("HSMD_SMD_MAX_CHUNK", str(max_chunk))
+    _say("fft form dssf of a fed trajectory, n_omega", no, "max chunk", max_chunk)
+    rng = np.random.default_rng(4061)
+    c, lats, vac = _config4_lattices(rng)
+    sub = [lats[3], lats[1], H.Lattice.generate("square", 10, c)]
+    idx = np.array([0, 1, 2, 0, 2], np.int32)
+    T, dt = 2560, 0.002
+    tt = np.arange(T)
+    w = np.exp(-((tt - T / 2) / (0.35 * T)) ** 2)
+    trajs = [_unit(rng, (T, sub[i].N, 3)) for i in idx]
+    gms = []
+    for li, L in enumerate(sub):
+        g = L.grid_map().copy()
+        if li == 0:
+            g[(g >= 0) & (vac[np.maximum(g, 0)] != 0)] = -1
+        gms.append(g)
+    ref = [sum(cpu_ref.dssf_fft(trajs[k], gms[li], sub[li].Gsz, w, no)
+               for k in range(len(idx)) if idx[k] == li) for li in range(len(sub))]
+    smd = H.SpinDynamics(sub, idx, dt, T, n_omega=no, precision=H.BATCHED_F32, window=w)
+    smd.set_shape(0, 6)
+    _feed_blocks(smd, trajs)
+    for li in range(len(sub)):
+        d, _ = smd.dssf(li)
+        a = ref[li]
+        assert d.shape == a.shape
+        err = np.abs(d - a).max() / a.max()
+        pk = a.max(axis=2, keepdims=True)
+        big = a > 1e-2 * pk
+        rel = (np.abs(d - a)[big] / a[big]).max()
+        _say("lattice %d: max err / peak %.2e, max rel err above 1e-2 peak %.2e" % (li, err, rel))
+        assert err < 1e-5
+        assert rel < 1e-4
+
+
 @pytest.mark.parametrize("gemm", [5, 4, 0, 2, 3, 1])
 def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm, monkeypatch):
     """The same batch integrated by both engines: FP32 RK4 with the MFMA

@@ -138,9 +138,16 @@ def test_shape_errors():
         e.set_shape(0, 4)      # 2 n_omega = 128: no 512-column tile
     with pytest.raises(H.HSMDError):
         e.set_shape(0, 5)
+    with pytest.raises(H.HSMDError):
+        e.set_shape(0, 6)      # n_steps = 64: no FFT form (needs a multiple of 512)
+    with pytest.raises(H.HSMDError):
+        e.set_shape(0, 7)
     e64 = H.SpinDynamics([lat], np.zeros(2, np.int32), 0.01, 64, n_omega=64, precision=H.BATCHED_F64)
     with pytest.raises(H.HSMDError):
         e64.set_shape(1, 0)    # the 128 x 128 tile is FP32 only
+    e64b = H.SpinDynamics([lat], np.zeros(2, np.int32), 0.01, 512, n_omega=64, precision=H.BATCHED_F64)
+    with pytest.raises(H.HSMDError):
+        e64b.set_shape(0, 6)   # the FFT form is FP32 only
     mc = H.MonteCarlo(lat, [1.0], seed=1)
     with pytest.raises(H.HSMDError):
         mc.set_threads(100)

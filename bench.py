@@ -659,10 +659,9 @@ def bench_smd(timer, args, rng, tune):
         # per launch over the live launch time), HBM beside it
         troof = valu_roofline("dssf_fft_config4_pmc.txt", gemm_ms * 1e-3, tr_bytes,
                               GEMM_KERNEL[6] + " (one %d-point chunk)" % chunk,
-                              extra={"algorithmic_flop_per_launch": alg_flop,
-                                     "algorithmic_tflops": alg_flop / (gemm_ms * 1e-3) / 1e12,
-                                     "note": "T / 512 decimated 512-point FFTs per row pair "
-                                             "and one complex multiply-add per bin and t2"})
+                              extra={"note": "T / 512 decimated 512-point FFTs per row pair "
+                                             "and one complex multiply-add per bin and t2; "
+                                             "HBM: the chunk read once, Y written once"})
     else:
         troof = {"bound": "mfma",
                  "achieved": alg_flop / (gemm_ms * 1e-3) / 1e12,

@@ -641,6 +641,36 @@ def test_fft_form_of_a_known_trajectory_against_fp64(no, max_chunk, monkeypatch)
         assert rel < 1e-4
 
 
+@pytest.mark.parametrize("T,no", [(512, 512), (1536, 384)])
+def test_fft_form_short_runs_against_fp64(T, no):
+    """The FFT form at P = T / 512 = 1 (the transform is one 512-point FFT per
+    row pair, no combine over t2) and at an odd P = 3 with 384 bins: square
+    L=10 (300 rows) and kagome L=8 (576 rows), three samples, fed unit
+    vectors, Hann window, against the FP64 transform: within 1e-5 of the
+    peak and 1e-4 relative above 1e-2 of the (q, component) peak."""
+    _say("fft form short runs, T", T, "bins", no)
+    rng = np.random.default_rng(4062 + T)
+    c = H.Couplings(J1=1.0, J2=0.6)
+    sub = [H.Lattice.generate("square", 10, c), H.Lattice.generate("kagome", 8, c)]
+    idx = np.array([0, 1, 1], np.int32)
+    w = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(T) / T)
+    trajs = [_unit(rng, (T, sub[i].N, 3)) for i in idx]
+    ref = [sum(cpu_ref.dssf_fft(trajs[k], sub[li].grid_map(), sub[li].Gsz, w, no)
+               for k in range(len(idx)) if idx[k] == li) for li in range(len(sub))]
+    smd = H.SpinDynamics(sub, idx, 0.01, T, n_omega=no, precision=H.BATCHED_F32, window=w)
+    smd.set_shape(0, 6)
+    _feed_blocks(smd, trajs)
+    for li in range(len(sub)):
+        d, _ = smd.dssf(li)
+        a = ref[li]
+        err = np.abs(d - a).max() / a.max()
+        big = a > 1e-2 * a.max(axis=2, keepdims=True)
+        rel = (np.abs(d - a)[big] / a[big]).max()
+        _say("lattice %d: max err / peak %.2e, max rel err above 1e-2 peak %.2e" % (li, err, rel))
+        assert err < 1e-5
+        assert rel < 1e-4
+
+
 @pytest.mark.parametrize("gemm", [5, 4, 0, 2, 3, 1])
 def test_config4_dssf_across_chunks_fp32_vs_fp64(gemm, monkeypatch):
     """The same batch integrated by both engines: FP32 RK4 with the MFMA

@@ -127,6 +127,34 @@ def test_smd_tuner_and_shapes_keep_parity(tmp_path, monkeypatch):
             np.testing.assert_allclose(r[k], res[0][k], rtol=1e-3, atol=1e-6 * res[0][k].max())
 
 
+def test_smd_tuner_with_the_fft_form(tmp_path, monkeypatch):
+    """n_steps = 1024 (a multiple of 512) and 128 bins: the FFT form is the
+    default and a tuner candidate; the chosen shape and every transform form
+    give the same DSSF up to FP32 rounding."""
+    _say("smd tuner with the fft form")
+    monkeypatch.setenv("HSMD_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lats = [H.Lattice.generate("square", 16, c), H.Lattice.generate("hexagn", 6, c)]
+    idx = np.array([0, 1, 0, 1], np.int32)
+    rng = np.random.default_rng(19)
+    samples = [_spins(rng, 1, lats[i].N)[0] for i in idx]
+    T, dt = 1024, 0.005
+    sd = H.SpinDynamics(lats, idx, dt, T, n_omega=128, tune=True, tune_steps=16)
+    t = sd.tuning
+    assert t["form"] in (1, 2, 3) and t["gemm"] in (0, 1, 2, 3, 6)
+    assert t["gemm_ms"] <= t["default_gemm_ms"]
+    res = []
+    for gemm in (6, 0, t["gemm"]):
+        e = H.SpinDynamics(lats, idx, dt, T, n_omega=128)
+        e.set_shape(t["form"], gemm)
+        e.set_samples(samples)
+        e.step(T - 1)
+        res.append((e.dssf(0)[0], e.dssf(1)[0]))
+    for r in res[1:]:
+        for k in (0, 1):
+            np.testing.assert_allclose(r[k], res[0][k], rtol=1e-3, atol=1e-6 * res[0][k].max())
+
+
 def test_shape_errors():
     _say("shape errors")
     c = H.Couplings(J1=1.0, J2=0.6)

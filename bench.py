@@ -469,12 +469,10 @@ def bench_mc(timer, args, rng, tune):
     timer.start()
     mc.sweeps_only(ksteps)
     kms = timer.stop() / ksteps
-    thr = mc.tuning["threads"] if mc.tuning else 1024
     z = 2 * lat.n_bonds // N          # coupled slots per site (18 here)
-    # the eighteen-slot float4 replica statically placed in LDS (the workload
-    # here) when every colour class fits one round of the workgroup
-    mc_kernel = ("heis_sweep_fast<float,18,true,false,true,5>" if np.bincount(lat.colours()).max() <= thr
-                 else "heis_sweep_kernel<float,true,20>") + " x %d threads" % thr
+    # the kernel the engine launches, as the library names it (the eighteen-slot
+    # float4 replica statically placed in LDS, classes ordered per wave, here)
+    mc_kernel = mc.sweep_kernel()
     gather_bytes = (z + 2) * 12 + 4 * z   # own read+write, z spins, z indices
     # HBM bytes a sweep must move with each replica resident in LDS: one read
     # and one write of its spins (12 B per site) plus one pass over the packed

@@ -105,6 +105,7 @@ SIGNATURES = {
     "hsmd_lattice_set_qc_states": (_I, [_P, _D, _D, _I, _I32]),
     "hsmd_mc_qc_histograms": (_I, [_P, _I32, _I]),
     "hsmd_mc_set_threads": (_I, [_P, _I]),
+    "hsmd_mc_sweep_kernel": (_I, [_P, ctypes.c_char_p, _I]),
     "hsmd_mc_autotune": (_I, [_P, _I, _D, ctypes.POINTER(MCSpec), _I, ctypes.POINTER(MCTuning)]),
     "hsmd_smd_set_shape": (_I, [_P, _I, _I]),
     "hsmd_smd_set_samples_from_mc": (_I, [_P, _P, _I32]),
@@ -355,6 +356,14 @@ class MonteCarlo:
     def set_threads(self, threads):
         """Threads per replica workgroup of the batched forms."""
         _check(lib().hsmd_mc_set_threads(self._h, int(threads)))
+
+    def sweep_kernel(self):
+        """Name of the sweep kernel the engine launches in its current form
+        (ends in ' wave-ordered' when the fast kernel orders its colour
+        classes per wave instead of by a workgroup barrier)."""
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().hsmd_mc_sweep_kernel(self._h, buf, 256))
+        return buf.value.decode()
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -38,6 +38,7 @@ void mc_config_energies_noalloc(const MCEngine &e, int R, const double *spins_de
                                 double *energies_dev, hipStream_t st);
 void mc_launch_sweeps(MCEngine &e, int n, hipStream_t st);
 void mc_set_threads(MCEngine &e, int threads);
+std::string mc_sweep_kernel(const MCEngine &e);
 void mc_autotune(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp, int n_sweeps,
                  hsmd_mc_tuning *out);
 void fields_energies(const Lattice &L, int precision, int R, const double *spins, double *fields,
@@ -293,6 +294,14 @@ hsmd_status hsmd_mc_set_threads(hsmd_mc *mc, int threads) {
   return guarded([&] {
     REQUIRE(mc, "null argument");
     mc_set_threads(*mc->e, threads);
+  });
+}
+
+hsmd_status hsmd_mc_sweep_kernel(hsmd_mc *mc, char *name, int len) {
+  return guarded([&] {
+    REQUIRE(mc && name && len > 0, "null argument");
+    const std::string k = mc_sweep_kernel(*mc->e);
+    snprintf(name, (size_t)len, "%s", k.c_str());
   });
 }
 

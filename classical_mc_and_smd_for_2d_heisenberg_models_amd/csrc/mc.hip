@@ -97,6 +97,12 @@ struct SweepArgs {
   const uint8_t *wcnt;
   double shj[3], shjz[3];   // couplings per shell (FP32 copies below)
   float fshj[3], fshjz[3];
+  // fast kernels, wave-level ordering instead of the class barrier (wdep !=
+  // nullptr): wdep[col * 16 + w] = bit mask of the waves that own a
+  // neighbour (any class, 64 sites of a class per wave) of a site that wave
+  // w updates in class col; sync_err is set when a wait ran out
+  const uint32_t *wdep;
+  uint32_t *sync_err;
 };
 
 __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
@@ -660,6 +666,20 @@ void heis_sweep_fast(const SweepArgs a) {
       cen = a.col_off[1 % nc + 1];
     }
   }
+  // wave-level ordering (wdep != nullptr): wprog[w] = class iterations wave
+  // w has completed (spins written).  Before iteration it a wave waits until
+  // every wave that owns a neighbour of its sites has completed it - 1: then
+  // the neighbours it reads hold their values of iteration it - 1, and no
+  // wave still reading one of its sites for an earlier iteration remains, so
+  // each site reads exactly what it reads under a barrier per class.
+  __shared__ int wprog[16];
+  const bool wsync = a.wdep != nullptr;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  if (tid < 16) wprog[tid] = 0;
+  // this wave's masks, lane k holding class k's (at most 64 classes), so the
+  // class loop picks one with a lane read instead of a memory load
+  const uint32_t dmv = (wsync && lane < nc) ? a.wdep[lane * 16 + wv] : 0u;
   __syncthreads();
   // one colour class; the label registers alternate between two sets (the
   // loop runs two classes per trip), so no copies follow the barrier.  Lanes
@@ -682,6 +702,25 @@ void heis_sweep_fast(const SweepArgs a) {
     const int p = cb + tid;
     // compact labels: the words any site of this wave's 64-site chunk uses
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
+    if (wsync) {
+      const uint32_t dm = (uint32_t)__builtin_amdgcn_readlane((int)dmv, col);
+      if (dm) {
+        // bounded: a wait that runs out flags sync_err and goes on, so the
+        // kernel always ends (the host reports the flag)
+        uint32_t n = 0;
+        while (true) {
+          const int v = __hip_atomic_load(&wprog[lane & 15], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const bool behind = ((dm >> (lane & 15)) & 1u) && v < it;
+          if (__builtin_amdgcn_ballot_w64(behind) == 0) break;
+          if (++n > (1u << 20)) {
+            if (lane == 0) atomicOr(a.sync_err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
     if (p < ce) {
       T hx = T(0), hy = T(0), hz = hz0;
       // FP32: the site's own spin is read with the first group of
@@ -779,7 +818,12 @@ void heis_sweep_fast(const SweepArgs a) {
       else { sx[p] = x; sy[p] = y; sz[p] = z; }
     }
     if (pass >= nor) keep = !(col & 1) && p < ce;
-    __syncthreads();
+    if (wsync) {
+      // the release orders this wave's spin writes before the count
+      if (lane == 0) __hip_atomic_store(&wprog[wv], it + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      __syncthreads();
+    }
     cb = cbn;
     ce = cen;
     cbn = cb2;
@@ -789,6 +833,7 @@ void heis_sweep_fast(const SweepArgs a) {
     iteration(it, qa, la, qb, lb);
     if (it + 1 < total) iteration(it + 1, qb, lb, qa, la);
   }
+  if (wsync) __syncthreads();
   bool written = false;
   if constexpr (EW > 0) {
     // E = escale sum_i S_i . sum_{e < 2 EW} (ej_e S_nb(i, e)) - h sum_i z_i over
@@ -1208,6 +1253,7 @@ struct MCEngine {
   bool fast_ok = false;
   bool compact = false;   // fast kernels read compact labels (see SweepArgs::wcnt)
   DevBuf<uint8_t> wcnt;
+  DevBuf<uint32_t> wdep, sync_err;   // wave-level ordering of the fast kernels
   int maxcol = 0;
   bool aos = false;       // FP32 fast path with float4 sites in LDS
   bool stat = false;      // ... in the statically placed LDS array (large replicas)
@@ -1355,6 +1401,57 @@ static size_t lds_limit() { return 160 * 1024 - NOBS * 16 * 8 - (kBCMAX + 1) * 1
 
 template <int ZC>
 static void set_lds_attr(MCEngine &e);
+
+// Wave dependencies of the fast kernels: wave w updates sites [64 w, 64 w +
+// 64) of each colour class, so the owner of site j is its position in its
+// class over 64.  For each (class, wave) the mask holds the owners of every
+// label the kernel reads for that wave's sites (the packed table, compact or
+// full; the sentinel N belongs to no wave), without the wave itself.  The
+// fast kernels then order the classes per wave instead of per workgroup.
+// Used when the masks are local (on average at most a quarter of the waves;
+// generated lattices in site order: the waves either side), or as
+// HSMD_MC_WAVE_SYNC = 1 / 0 forces.
+static void build_wave_deps(MCEngine &e, const std::vector<uint32_t> &pk, int words) {
+  const Lattice &lat = e.lat;
+  const int N = lat.N, nc = lat.n_colours;
+  SweepArgs &a = e.args;
+  a.wdep = nullptr;
+  a.sync_err = nullptr;
+  std::vector<int> owner(N + 1, -1);
+  for (int col = 0; col < nc; col++)
+    for (int p = lat.col_off[col]; p < lat.col_off[col + 1]; p++) owner[p] = (p - lat.col_off[col]) >> 6;
+  const uint32_t lmask = e.compact ? 0x3fffu : 0xffffu;
+  std::vector<uint32_t> dep((size_t)nc * 16, 0u);
+  for (int col = 0; col < nc; col++)
+    for (int p = lat.col_off[col]; p < lat.col_off[col + 1]; p++) {
+      const int w = owner[p];
+      if (w < 0 || w > 15) return;   // a class beyond 1024 sites: no fast kernel
+      uint32_t &m = dep[(size_t)col * 16 + w];
+      for (int k = 0; k < words; k++) {
+        const uint32_t word = pk[(size_t)k * N + p];
+        const uint32_t l0 = word & lmask, l1 = (word >> 16) & lmask;
+        if ((int)l0 < N && owner[l0] >= 0) m |= 1u << owner[l0];
+        if ((int)l1 < N && owner[l1] >= 0) m |= 1u << owner[l1];
+      }
+      m &= ~(1u << w);
+    }
+  size_t bits = 0, used = 0;
+  for (int col = 0; col < nc; col++) {
+    const int nw = (lat.col_off[col + 1] - lat.col_off[col] + 63) / 64;
+    for (int w = 0; w < nw; w++) {
+      bits += __builtin_popcount(dep[(size_t)col * 16 + w]);
+      used++;
+    }
+  }
+  bool on = used > 0 && bits <= 4 * used;   // mean mask size <= 4 waves
+  if (const char *ev = getenv("HSMD_MC_WAVE_SYNC")) on = atoi(ev) != 0;
+  if (!on) return;
+  e.wdep.upload(dep.data(), dep.size());
+  e.sync_err.alloc(1);
+  e.sync_err.zero();
+  a.wdep = e.wdep.p;
+  a.sync_err = e.sync_err.p;
+}
 
 MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp) {
   if (R < 1 || R > kThreads) throw Error(HSMD_ERR_ARG, "replica count must be in [1, 1024]");
@@ -1562,6 +1659,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
       }
       e->nb16.upload(pk.data(), pk.size());
       a.nb16 = e->nb16.p;
+      build_wave_deps(*e, pk, (zc_for(lat.Z) + 1) / 2);
       if (!e->compact) pick_energy_form(lat, a.cp, a);
       a.nbE = nullptr;
       a.nEw = 0;
@@ -1627,7 +1725,17 @@ void mc_set_spins(MCEngine &e, const double *spins) {
   e.spins_set = true;
 }
 
+// a fast-kernel wave wait that ran out (never expected: the waits end
+// once every wave has written its classes) makes the results invalid
+static void check_wave_sync(MCEngine &e) {
+  if (!e.args.sync_err) return;
+  uint32_t f = 0;
+  HIP_CHECK(hipMemcpy(&f, e.sync_err.p, sizeof f, hipMemcpyDeviceToHost));
+  if (f) throw Error(HSMD_ERR_DEVICE, "sweep kernel: a wave dependency wait ran out");
+}
+
 void mc_get_spins(MCEngine &e, double *spins) {
+  check_wave_sync(e);
   const Lattice &L = e.lat;
   const int N = L.N, R = e.R;
   const size_t Npad = (size_t)N + 1;
@@ -1743,6 +1851,41 @@ static void launch_sweep(MCEngine &e, hipStream_t st) {
     return;
   }
   HSMD_ZC_DISPATCH(e.lat.Z, launch_sweep_zc, e, st);
+}
+
+// The sweep kernel launch_sweep starts for the engine's current form, as
+// "<kernel><template arguments> x <threads>", and " wave-ordered" when the
+// fast kernel orders its classes per wave instead of per workgroup.
+std::string mc_sweep_kernel(const MCEngine &e) {
+  const SweepArgs &a = e.args;
+  const bool f32 = e.prec == HSMD_BATCHED_F32;
+  const int ZC = zc_for(e.lat.Z);
+  const bool passes = a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0;
+  const char *T = f32 ? "float" : "double";
+  char b[160];
+  if (e.prec == HSMD_PLAIN_F64) {
+    snprintf(b, sizeof b, "%s<%s%d> x %d, one replica per launch",
+             e.ising ? "ising_sweep_kernel" : "heis_sweep_kernel", e.ising ? "false," : "double,false,", ZC,
+             kThreads);
+    return b;
+  }
+  bool fast = false;
+  if (e.ising) {
+    snprintf(b, sizeof b, "ising_sweep_kernel<%s,%d> x %d", e.use_lds ? "true" : "false", ZC, e.threads);
+  } else if (e.stat && e.fast && !e.compact && passes && e.lat.Z > 16 && e.lat.Z <= 18) {
+    snprintf(b, sizeof b, "heis_sweep_fast<float,18,true,false,true%s> x %d",
+             a.table_energy && a.nEw == 5 ? ",5" : "", e.threads);
+    fast = true;
+  } else if (e.fast && passes) {
+    snprintf(b, sizeof b, "heis_sweep_fast<%s,%d,%s,%s> x %d", T, ZC, f32 && e.aos ? "true" : "false",
+             e.compact ? "true" : "false", e.threads);
+    fast = true;
+  } else {
+    snprintf(b, sizeof b, "heis_sweep_kernel<%s,%s,%d> x %d", T, e.use_lds ? "true" : "false", ZC, e.threads);
+  }
+  std::string r = b;
+  if (fast && a.wdep) r += " wave-ordered";
+  return r;
 }
 
 static void launch_obs(MCEngine &e, hipStream_t st) {
@@ -1870,6 +2013,7 @@ void mc_run(MCEngine &e, int n_sweeps, int pt_parity, int measure, double *etrac
 }
 
 void mc_energies(MCEngine &e, double *E) {
+  check_wave_sync(e);
   std::vector<double> ec(e.R);
   std::vector<int32_t> cos(e.R);
   HIP_CHECK(hipMemcpy(ec.data(), e.energy.p, e.R * sizeof(double), hipMemcpyDeviceToHost));

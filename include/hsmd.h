@@ -198,6 +198,12 @@ hsmd_status hsmd_mc_set_threads(hsmd_mc *mc, int threads);
 hsmd_status hsmd_mc_autotune(const hsmd_lattice *lat, int R, const double *temps,
                              const hsmd_mc_spec *spec, int n_sweeps, hsmd_mc_tuning *out);
 hsmd_status hsmd_mc_get_spins(hsmd_mc *mc, double *spins);
+/* Name of the sweep kernel the engine launches in its current form (kernel,
+ * template arguments, threads per workgroup; " wave-ordered" when the fast
+ * kernel orders its colour classes per wave rather than by a workgroup
+ * barrier), at most len bytes with the terminator.  Introspection with no
+ * counterpart in the reference. */
+hsmd_status hsmd_mc_sweep_kernel(hsmd_mc *mc, char *name, int len);
 
 /* Runs n_sweeps sweeps of every replica; replaces the sweep loops
  * CF/mc_hm_code.cpp:2778-2962 (thermalisation) and 4836-4945 (measurement)

@@ -60,9 +60,6 @@ GEMM_KERNEL = {0: "time_gemm_mfma_q_kernel", 1: "time_gemm_mfma_kernel",
                2: "time_gemm_mfma_q256_kernel", 3: "time_gemm_mfma_db_kernel",
                4: "time_gemm_mfma_w_kernel", 5: "time_gemm_mfma_p_kernel",
                6: "time_dft_fft_kernel"}
-# RK4 kernel of the configuration-4 batch (slot classes 8 and 12) per SMD form
-SMD_FORM_KERNEL = {1: "smd_kernel<float,4,true,12,false>", 2: "smd_kernel_two<float,4,8,12>",
-                   3: "smd_kernel_wide<float,12,false>"}
 
 
 class HipTimer:
@@ -607,15 +604,17 @@ def bench_smd(timer, args, rng, tune):
     chunk = min(T, SMD_CHUNK)    # FP32 trajectory chunk (memory permitting)
     smd = H.SpinDynamics(lats, idx, 0.005, T, n_omega=512, precision=H.BATCHED_F32,
                          tune=tune, tune_steps=64)
-    # breakdown run (also the warm-up): the RK4 kernel alone over 1000 steps
-    # (one launch, no chunk fills), then the step that fills the chunk (one
-    # RK4 step + the contraction of the whole chunk) against a plain step
+    # breakdown run (also the warm-up): two steps (t = 0 recorded, kernels
+    # loaded), the RK4 kernel alone over 1000 steps (one launch, no chunk
+    # fills), then the step that fills the chunk (one RK4 step + the
+    # contraction of the whole chunk) against a plain step
     smd.set_samples(samples)
-    nb = min(1000, chunk - 3)
+    smd.step(2)
+    nb = min(1000, chunk - 5)
     timer.start()
     smd.step(nb)
     rk_ms = timer.stop() / nb
-    smd.step(chunk - 2 - nb)
+    smd.step(chunk - 4 - nb)
     timer.start()
     smd.step(1)
     t_flush = timer.stop()
@@ -639,10 +638,9 @@ def bench_smd(timer, args, rng, tune):
     # HBM bytes one step must move with every sample resident in LDS: the
     # FP32 trajectory row the contraction reads back (3 N values per sample)
     rk_hbm = sum(4 * 3 * lats[i].N for i in idx)
-    form = smd.tuning["form"] if smd.tuning else 2
     gemm = smd.tuning["gemm"] if smd.tuning else 6
     roof = valu_roofline("smd_rk4_f32_config4_pmc.txt", rk_ms * 1e-3, rk_hbm,
-                         SMD_FORM_KERNEL.get(form, "?") + " (%d steps per launch)" % nb,
+                         smd.step_kernel() + " (%d steps per launch)" % nb,
                          unit="step",
                          extra={"per": "RK4 step of the 512-sample batch",
                                 "gather_model": {"bytes_per_step": rk_gather,

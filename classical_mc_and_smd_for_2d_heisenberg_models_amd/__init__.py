@@ -108,6 +108,7 @@ SIGNATURES = {
     "hsmd_mc_sweep_kernel": (_I, [_P, ctypes.c_char_p, _I]),
     "hsmd_mc_autotune": (_I, [_P, _I, _D, ctypes.POINTER(MCSpec), _I, ctypes.POINTER(MCTuning)]),
     "hsmd_smd_set_shape": (_I, [_P, _I, _I]),
+    "hsmd_smd_step_kernel": (_I, [_P, ctypes.c_char_p, _I]),
     "hsmd_smd_set_samples_from_mc": (_I, [_P, _P, _I32]),
     "hsmd_smd_autotune": (_I, [ctypes.POINTER(_P), _I, _I32, _I, ctypes.POINTER(SMDSpec), _D, _I,
                                ctypes.POINTER(SMDTuning)]),
@@ -489,6 +490,14 @@ class SpinDynamics:
         128 x 512 tile pipelined, same sums as 4) or 6 (the time DFT as a
         pruned FFT: FP32, n_steps a multiple of 512, n_omega <= 512)."""
         _check(lib().hsmd_smd_set_shape(self._h, int(form), int(gemm)))
+
+    def step_kernel(self):
+        """Name of the RK4 kernel step() launches in the current form (ends
+        in ', wave-ordered on k of n lattices' when the LDS-resident form
+        orders its stages per wave instead of by workgroup barriers)."""
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().hsmd_smd_step_kernel(self._h, buf, 256))
+        return buf.value.decode()
 
     def set_samples_from_mc(self, mc, slots):
         """Sample s starts from the configuration at temperature slot

@@ -57,6 +57,7 @@ void smd_dssf_q(SMDEngine &e, int li, int nq, const double *q, double *out);
 void smd_energies(SMDEngine &e, double *E);
 void smd_destroy(SMDEngine *e);
 void smd_set_shape(SMDEngine &e, int form, int gemm);
+std::string smd_step_kernel(const SMDEngine &e);
 McHandoff mc_handoff(MCEngine &e);
 void smd_set_samples_from_mc(SMDEngine &e, const McHandoff &h, const int32_t *slots);
 void smd_autotune(const Lattice *const *lats, int nl, const int32_t *sample_lattice, int S,
@@ -409,6 +410,14 @@ hsmd_status hsmd_smd_set_samples_from_mc(hsmd_smd *smd, hsmd_mc *mc, const int32
   return guarded([&] {
     REQUIRE(smd && mc && slots, "null argument");
     smd_set_samples_from_mc(*smd->e, mc_handoff(*mc->e), slots);
+  });
+}
+
+hsmd_status hsmd_smd_step_kernel(hsmd_smd *smd, char *name, int len) {
+  return guarded([&] {
+    REQUIRE(smd && name && len > 0, "null argument");
+    const std::string k = smd_step_kernel(*smd->e);
+    snprintf(name, (size_t)len, "%s", k.c_str());
   });
 }
 

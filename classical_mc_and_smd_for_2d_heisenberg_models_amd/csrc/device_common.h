@@ -198,4 +198,27 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch) {
   __syncthreads();
 }
 
+// block_sum with one barrier: for callers that alternate between two scratch
+// halves (a half is written again only after every wave has passed the
+// barrier of the call in between, so no wave still reads it) and need no
+// barrier after the sums.
+template <int NV>
+__device__ __forceinline__ void block_sum_once(double (&v)[NV], double *scratch) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) scratch[k * 16 + w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += scratch[k * 16 + i];
+    v[k] = s;
+  }
+}
+
 }  // namespace hsmd

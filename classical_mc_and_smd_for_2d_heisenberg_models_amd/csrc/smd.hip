@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <string>
 
 #include "device_common.h"
 #include "hsmd_internal.h"
@@ -88,6 +89,38 @@ __global__ __launch_bounds__(256) void time_gemm_kernel(const SampleDev *samples
 #pragma unroll
     for (int j = 0; j < 4; j++) Yb[(size_t)m * ldb + j0 + tx * 4 + j] += acc[i][j];
   }
+}
+
+// Wave dependencies of the LDS-resident RK4 form: site p is integrated by
+// thread p mod 1024, so by wave (p mod 1024) / 64 at any sites-per-thread
+// count; wdep[w] = the waves owning a neighbour (any coupled or padded slot
+// the kernel reads) of a site of wave w, without w.  The stages are ordered
+// per wave when the masks are local (on average at most eight waves; lattices
+// in the reference's site order: the waves either side), or as
+// HSMD_SMD_WAVE_SYNC = 1 / 0 forces.
+static void set_wave_deps(const Lattice &L, LatDev &d) {
+  const int N = L.N;
+  for (int w = 0; w < 16; w++) d.wdep[w] = 0u;
+  bool used[16] = {false};
+  for (int p = 0; p < N; p++) {
+    const int w = (p & 1023) >> 6;
+    used[w] = true;
+    for (int k = 0; k < L.Z; k++) {
+      const int q = L.g_nb[(size_t)k * N + p];
+      if (q >= 0 && q < N) d.wdep[w] |= 1u << ((q & 1023) >> 6);
+    }
+  }
+  size_t bits = 0, nw = 0;
+  for (int w = 0; w < 16; w++) {
+    d.wdep[w] &= ~(1u << w);
+    if (used[w]) {
+      bits += __builtin_popcount(d.wdep[w]);
+      nw++;
+    }
+  }
+  bool on = nw > 0 && bits <= 8 * nw;
+  if (const char *ev = getenv("HSMD_SMD_WAVE_SYNC")) on = atoi(ev) != 0;
+  d.wsync = on ? 1 : 0;
 }
 
 // --------------------------------------------------- spatial transform
@@ -213,6 +246,8 @@ struct SMDEngine {
   DevBuf<SampleDev> dsamples;
   DevBuf<unsigned char> spins, s0, stage, traj, ybuf, twid, dftcs;
   DevBuf<double> qss, eref, dssf, dd;
+  DevBuf<uint32_t> sync_err;   // wave-ordering waits of the LDS form that ran out
+  std::vector<char> lat_ws;     // per lattice: stages ordered per wave in the LDS form
   DevBuf<unsigned char> ks;
   DevBuf<int> sample_ids;
   DevBuf<GemmSampleHost> gemm_samples;
@@ -466,6 +501,8 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
       }
     }
     d.field_energy = L.field_energy_ok() ? 1 : 0;
+    set_wave_deps(L, d);
+    e->lat_ws.push_back((char)(d.wsync && d.field_energy));
     // DFT grid in new labels; vacant sites are masked out of the transform
     if (!L.gridMap.empty()) {
       std::vector<int32_t> g(L.gridMap.size());
@@ -557,6 +594,8 @@ SMDEngine *smd_create(const Lattice *const *lats, int nl, const int32_t *sample_
   e->qss.alloc((size_t)S * T_);
   e->qss.zero();
   e->eref.alloc(S);
+  e->sync_err.alloc(1);
+  e->sync_err.zero();
   if (sp.dssf) {
     e->traj.alloc(to * es);
     e->traj.zero();   // the pad columns of each row stay zero
@@ -801,6 +840,7 @@ void smd_step(SMDEngine &e, int n) {
     a.stage_stride = (long long)(e.stage.n / (2 * (e.f32 ? sizeof(float) : sizeof(double))));
     a.ks = e.ks.p;
     a.dd = e.dd.p;
+    a.sync_err = e.sync_err.p;
     const SmdLaunch lc = {e.S, e.spt, e.maxZ, e.minZ, e.general, e.use_lds, e.wide,
                           e.spec.precision == HSMD_PLAIN_F64, e.lds_bytes, e.one_class};
     if (e.f32) smd_launch_f32(lc, a); else smd_launch_f64(lc, a);
@@ -811,6 +851,9 @@ void smd_step(SMDEngine &e, int n) {
     if (e.chunk_fill == e.Tc) flush_chunk(e);
   }
   HIP_CHECK(hipDeviceSynchronize());
+  uint32_t f = 0;
+  HIP_CHECK(hipMemcpy(&f, e.sync_err.p, sizeof f, hipMemcpyDeviceToHost));
+  if (f) throw Error(HSMD_ERR_DEVICE, "RK4 kernel: a wave-ordering wait ran out");
 }
 
 // Appends n time points of given configurations to every sample's
@@ -1096,6 +1139,39 @@ void smd_destroy(SMDEngine *e) { delete e; }
 static bool two_class(const SMDEngine &e) {
   if (!e.f32 || !e.lds_ok || e.general || e.spt > 4) return false;
   return (e.minZ <= 8 && e.maxZ > 8 && e.maxZ <= 12) || (e.minZ <= 4 && e.maxZ > 4 && e.maxZ <= 8);
+}
+
+// The RK4 kernel smd_step launches for the engine's current form, as
+// "<kernel><template arguments> x 1024", with ", wave-ordered on k of n
+// lattices" when the LDS-resident form orders its stages per wave.
+std::string smd_step_kernel(const SMDEngine &e) {
+  const char *T = e.f32 ? "float" : "double";
+  const bool plain = e.spec.precision == HSMD_PLAIN_F64;
+  const int Z = e.maxZ;
+  const int zc = e.general ? 32 : Z <= 4 ? 4 : Z <= 8 ? 8 : Z <= 12 ? 12 : Z <= 20 ? 20 : 32;
+  const int spt = e.spt <= 4 ? e.spt : 8;
+  const char *gen = e.general ? "true" : "false";
+  char b[200];
+  if (e.wide) {
+    snprintf(b, sizeof b, "smd_kernel_wide<%s,%d,%s> x 1024%s", T, zc, gen, plain ? ", one sample per launch" : "");
+    return b;
+  }
+  if (!e.use_lds) {
+    snprintf(b, sizeof b, "smd_kernel<%s,%d,false,%d,%s> x 1024, one sample per launch", T, spt, zc, gen);
+    return b;
+  }
+  int lo = 0, hi = 0;
+  if (e.f32 && spt <= 4 && !e.one_class && !e.general) {
+    if (e.minZ <= 8 && Z > 8 && Z <= 12) { lo = e.minZ <= 4 ? 4 : 8; hi = 12; }
+    else if (e.minZ <= 4 && Z > 4 && Z <= 8) { lo = 4; hi = 8; }
+  }
+  if (hi) snprintf(b, sizeof b, "smd_kernel_two<float,%d,%d,%d> x 1024", spt, lo, hi);
+  else snprintf(b, sizeof b, "smd_kernel<%s,%d,true,%d,%s> x 1024", T, spt, zc, gen);
+  std::string r = b;
+  int k = 0;
+  for (char c : e.lat_ws) k += c ? 1 : 0;
+  if (k) r += ", wave-ordered on " + std::to_string(k) + " of " + std::to_string(e.lat_ws.size()) + " lattices";
+  return r;
 }
 
 static int current_form(const SMDEngine &e) {

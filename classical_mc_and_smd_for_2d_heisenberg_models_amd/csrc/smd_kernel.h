@@ -26,6 +26,12 @@ struct LatDev {
   const uint32_t *nb16c;
   const uint8_t *wcnt;
   double shj[3], shjz[3];  // couplings per shell (J, J lambda)
+  // LDS-resident form with field energies: the RK stages ordered per wave
+  // (wsync != 0) instead of by workgroup barriers; wdep[w] = bit mask of the
+  // waves owning a neighbour of a site wave w integrates (site p belongs to
+  // thread p mod 1024 at any sites-per-thread count)
+  int wsync;
+  uint32_t wdep[16];
 };
 
 // Load through a global-address-space pointer (a plain global load rather
@@ -123,6 +129,7 @@ struct StepArgs {
   int T;                // total time points
   int lds_npad;         // LDS component stride, SPT x 1024 + 1 (LDS form)
   int sample_base;      // first sample of this launch
+  uint32_t *sync_err;   // set when a wave-ordering wait ran out
 };
 
 // Per-lattice constants the time loop reads.  The kernel copies them out of
@@ -281,8 +288,19 @@ __device__ __forceinline__ void field16(const FieldCtx<T, ZC> &c, const uint32_t
 // When the tables are symmetric (field_energy), E_ref and E(S) come from the
 // fields the step computes anyway, E = 1/2 sum_i S_i.h_i - 1/2 h sum_i S_z,i;
 // otherwise from the bond list.
+// Wave ordering (LDS form with field energies, lattice flag wsync): each
+// of the five neighbour-reading phases of a step (stages 1-4 and the
+// correction's fields) and the next step's stage 1 start once the waves that
+// own the phase's neighbours have published the phase before (an LDS count
+// per wave, released after the wave's stores).  A wave's dependencies also
+// cover every reader of the sites it writes (the neighbour relation is
+// symmetric), so no stage buffer is overwritten while a slower wave still
+// reads it; the per-step FP64 sums keep one workgroup barrier, with the
+// reduction scratch alternating between two halves.  Each site reads the
+// same values as under a barrier per phase.
 template <typename T, int SPT, bool LDS, int ZC, bool GEN>
-__device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem, double *scratch) {
+__device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem, double *scratch,
+                                         int *wprog) {
   constexpr int ZH = (ZC + 1) / 2;
   constexpr bool QREG = SPT * ZH <= 16;   // labels in registers
   constexpr bool GREG = SPT <= 2;         // t = 0 spins in registers
@@ -294,6 +312,10 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
   const uint32_t *nb16 = L.nb16;
   const bool fe = L.field_energy != 0;
   const double hext = L.cp.h_on ? L.cp.h : 0.0;
+  const bool ws = LDS && fe && L.wsync != 0;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const uint32_t dm = ws ? L.wdep[wv] : 0u;
   T *gS = (T *)a.spins + sd.spin_off;
   const T *g0 = (const T *)a.s0 + sd.spin_off;
   T *S, *B1, *B2;
@@ -334,7 +356,33 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
       for (int d = 0; d < 3; d++) G0[u][d] = p < N ? g0[d * Npad + p] : T(0);
     }
   }
+  if (threadIdx.x < 16) wprog[threadIdx.x] = 0;
   __syncthreads();
+  int wcount = 0;   // phases this wave has published
+  auto sync = [&]() __attribute__((always_inline)) {
+    if (ws) {
+      wcount++;
+      if (lane == 0) __hip_atomic_store(&wprog[wv], wcount, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (dm) {
+        // bounded: a wait that runs out flags sync_err and goes on
+        uint32_t n = 0;
+        while (true) {
+          const int v = __hip_atomic_load(&wprog[lane & 15], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const bool behind = ((dm >> (lane & 15)) & 1u) && v < wcount;
+          if (__builtin_amdgcn_ballot_w64(behind) == 0) break;
+          if (++n > (1u << 20)) {
+            if (lane == 0) atomicOr(a.sync_err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+      __syncthreads();
+    }
+  };
+  int half = 0;   // reduction scratch half of the wave-ordered form
   T *Sx = S, *Sy = S + stride, *Sz = S + 2 * stride;
   T *Px = B1, *Py = B1 + stride, *Pz = B1 + 2 * stride;
   T *Qx = B2, *Qy = B2 + stride, *Qz = B2 + 2 * stride;
@@ -390,7 +438,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           Pz[p] = z + T(0.5) * K1[u][2];
         }
       }
-      __syncthreads();
+      sync();
       // stage 2: K2 from B1, B2 = S + K2/2
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
@@ -406,7 +454,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           Qz[p] = Sz[p] + T(0.5) * A2[u][2];
         }
       }
-      __syncthreads();
+      sync();
       // stage 3: K3 from B2, B1 = S + K3, A2 = K2 + K3
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
@@ -424,7 +472,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           for (int d = 0; d < 3; d++) A2[u][d] += k3[d];
         }
       }
-      __syncthreads();
+      sync();
       // stage 4 and combine, S written in place
 #pragma unroll
       for (int u = 0; u < SPT; u++) {
@@ -443,7 +491,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           unit_store<T>(v, Sx[p], Sy[p], Sz[p]);
         }
       }
-      __syncthreads();
+      sync();
       // energy correction: sum |h x S|^2 and E(S) in FP64
       double D[SPT][3];
       double v4[4] = {0.0, 0.0, eref_part, qss_prev};
@@ -462,7 +510,12 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
         }
       }
       if (!fe) v4[1] = bond_energy<T>(L, Sx, Sy, Sz) + field_energy_term<T>(L, Sz);
-      block_sum<4>(v4, scratch);   // ends with a barrier: S may be overwritten below
+      if (ws) {
+        block_sum_once<4>(v4, scratch + half * 4 * 16);   // its barrier: S may be overwritten below
+        half ^= 1;
+      } else {
+        block_sum<4>(v4, scratch);   // ends with a barrier: S may be overwritten below
+      }
       if (fe) Eref = v4[2];
       if (t_prev >= 0 && threadIdx.x == 0) a.qss[(size_t)sidx * a.T + t_prev] = v4[3] / (double)N;
       const double cfac = (Eref - v4[1]) / v4[0];
@@ -483,7 +536,7 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
           if (row) { row[p] = tx; row[N + p] = ty; row[2 * N + p] = tz; }
         }
       }
-      __syncthreads();
+      sync();
       qss_prev = qs;
       t_prev = t;
       if (!fe) {
@@ -535,8 +588,9 @@ __device__ __forceinline__ void smd_body(const StepArgs &a, unsigned char *smem,
 template <typename T, int SPT, bool LDS, int ZC, bool GEN>
 __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double scratch[4 * 16];
-  smd_body<T, SPT, LDS, ZC, GEN>(a, smem, scratch);
+  __shared__ double scratch[2 * 4 * 16];
+  __shared__ int wprog[16];
+  smd_body<T, SPT, LDS, ZC, GEN>(a, smem, scratch, wprog);
 }
 
 // Mixed batches whose lattices fall in two slot classes: each workgroup runs
@@ -545,10 +599,11 @@ __global__ __launch_bounds__(kSmdThreads) void smd_kernel(const StepArgs a) {
 template <typename T, int SPT, int ZLO, int ZHI>
 __global__ __launch_bounds__(kSmdThreads) void smd_kernel_two(const StepArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double scratch[4 * 16];
+  __shared__ double scratch[2 * 4 * 16];
+  __shared__ int wprog[16];
   const SampleDev sd = a.samples[blockIdx.x + a.sample_base];
-  if (a.lats[sd.li].cp.Z <= ZLO) smd_body<T, SPT, true, ZLO, false>(a, smem, scratch);
-  else smd_body<T, SPT, true, ZHI, false>(a, smem, scratch);
+  if (a.lats[sd.li].cp.Z <= ZLO) smd_body<T, SPT, true, ZLO, false>(a, smem, scratch, wprog);
+  else smd_body<T, SPT, true, ZHI, false>(a, smem, scratch, wprog);
 }
 
 // Energy of every sample (FP64 reduction), used to seed E_ref.

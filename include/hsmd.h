@@ -335,6 +335,12 @@ typedef struct hsmd_smd_tuning {
  * Errors when the shape does
  * not apply to the batch. */
 hsmd_status hsmd_smd_set_shape(hsmd_smd *smd, int form, int gemm);
+/* Name of the RK4 kernel hsmd_smd_step launches in the engine's current form
+ * (kernel, template arguments, threads per workgroup; ", wave-ordered on k of
+ * n lattices" when the LDS-resident form orders its RK stages per wave
+ * rather than by workgroup barriers), at most len bytes with the terminator.
+ * Introspection with no counterpart in the reference. */
+hsmd_status hsmd_smd_step_kernel(hsmd_smd *smd, char *name, int len);
 /* Autotuner of the batched SMD step (no counterpart in the reference): on a
  * scratch engine of the same batch, times n_steps RK4 steps per applicable
  * form and one trajectory-chunk contraction per tile, and returns the

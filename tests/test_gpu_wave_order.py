@@ -120,3 +120,72 @@ def test_poisson_graph_compact_labels_wave_ordered_equal_barrier(monkeypatch):
     s = _unit(rng, (128, lat.N, 3))
     a = _compare(monkeypatch, lat, temps, s, 2, precision=H.BATCHED_F32, n_or=5, n_hb=1, seed=6)
     assert ",true> x" in a[0]
+
+
+# ------------------------------------------------------------ SMD RK4
+
+def _smd_chain(monkeypatch, mode, lats, idx, samples, T, form=None, **kw):
+    monkeypatch.setenv("HSMD_SMD_WAVE_SYNC", mode)
+    smd = H.SpinDynamics(lats, idx, 0.005, T, **kw)
+    if form:
+        smd.set_shape(form, 1)
+    name = smd.step_kernel()
+    smd.set_samples(samples)
+    smd.step(T - 1)
+    out = [name, smd.get_samples(), smd.energies()]
+    if kw.get("dssf", True):
+        for li in range(len(lats)):
+            out.extend(smd.dssf(li))
+    del smd
+    return out
+
+
+def _smd_compare(monkeypatch, lats, idx, samples, T, form=None, **kw):
+    a = _smd_chain(monkeypatch, "1", lats, idx, samples, T, form, **kw)
+    b = _smd_chain(monkeypatch, "0", lats, idx, samples, T, form, **kw)
+    _say(a[0], "|", b[0])
+    assert "wave-ordered on %d of %d lattices" % (len(lats), len(lats)) in a[0], a[0]
+    assert "wave-ordered" not in b[0], b[0]
+    assert len(a[1]) == len(b[1]) == len(idx)
+    for x, y in zip(a[1], b[1]):   # samples of uneven size
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a[2:], b[2:]):
+        np.testing.assert_array_equal(x, y)
+    assert np.isfinite(a[2]).all()
+    return a
+
+
+@pytest.mark.parametrize("form", [2, 1])
+def test_config4_wave_ordered_rk4_equals_barrier_rk4(monkeypatch, form):
+    """The four configuration-4 lattices (square L=64, kagome L=32,
+    hexagonal L=40, square L=48 with 230 vacancies), two FP32 samples each,
+    four sites per thread: per-workgroup slot classes (smd_kernel_two) and
+    the batch's largest class; 1024 RK4 steps (a launch boundary at 1000),
+    trajectories, energies, qSS and the 64-bin DSSF bit-identical."""
+    _say("config4 smd wave order, form", form)
+    rng = np.random.default_rng(411)
+    c = H.Couplings(J1=1.0, J2=0.6)
+    lats = [H.Lattice.generate("square", 64, c), H.Lattice.generate("kagome", 32, c),
+            H.Lattice.generate("hexagn", 40, c), H.Lattice.generate("square", 48, c)]
+    vac = np.zeros(lats[3].N, np.int32)
+    vac[rng.choice(lats[3].N, 230, replace=False)] = 1
+    lats[3].set_vacancies(vac)
+    idx = np.array([0, 1, 2, 3, 3, 2, 1, 0], np.int32)
+    samples = [_unit(rng, (lats[i].N, 3)) for i in idx]
+    a = _smd_compare(monkeypatch, lats, idx, samples, 1025, form=form, n_omega=64,
+                     precision=H.BATCHED_F32)
+    assert ("smd_kernel_two<float,4,8,12>" in a[0]) == (form == 2), a[0]
+
+
+@pytest.mark.parametrize("prec", [H.BATCHED_F32, H.BATCHED_F64])
+def test_small_lattices_wave_ordered_rk4_equals_barrier_rk4(monkeypatch, prec):
+    """Square L=24 (one site per thread: the configuration-2 lattice) and
+    triangular L=36 (1296 sites, two per thread; FP64 with both in one
+    batch at two sites per thread), 300 RK4 steps with the 32-bin DSSF."""
+    _say("small lattices smd wave order", prec)
+    rng = np.random.default_rng(412)
+    lats = [H.Lattice.generate("square", 24, H.Couplings(J1=1.0, J2=0.6)),
+            H.Lattice.generate("triang", 36, H.Couplings(J1=1.0, J2=0.12, JX=0.05))]
+    idx = np.array([0, 1, 1, 0], np.int32)
+    samples = [_unit(rng, (lats[i].N, 3)) for i in idx]
+    _smd_compare(monkeypatch, lats, idx, samples, 301, n_omega=32, precision=prec)

@@ -559,7 +559,8 @@ def bench_mc(timer, args, rng, tune):
     out["ising"] = {"value": R * N * isw / (ims * 1e-3), "unit": "spin-updates/s",
                     "ms_per_step": ims / isw, "steps": isw,
                     "workload": "configuration 3 graph, 256 replicas, one Metropolis pass + "
-                                "replica exchange per sweep, int8 spins"}
+                                "replica exchange per sweep, int8 spins",
+                    "kernel": mi.sweep_kernel()}
     print("[bench] MC ising: %.3f ms/sweep" % (ims / isw), file=sys.stderr, flush=True)
     del mi
     return out, (c, lat, temps)

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <tuple>
 
 #include "device_common.h"
 #include "hsmd_internal.h"
@@ -103,6 +104,11 @@ struct SweepArgs {
   // w updates in class col; sync_err is set when a wait ran out
   const uint32_t *wdep;
   uint32_t *sync_err;
+  // fast Ising kernel: the energy's integer bond counts from nbE when
+  // itable != 0, icls[e] = bond class of emask slot e (slot order, -1 for
+  // the padding)
+  int itable;
+  int icls[16];
 };
 
 __device__ __forceinline__ uint32_t sweep_of(const SweepArgs &a) {
@@ -1044,6 +1050,180 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_kernel(const SweepArgs a
   }
 }
 
+// Fast Ising sweep (one Metropolis pass): the int8 replica in LDS, uniform
+// couplings per slot, every colour class no larger than the workgroup (one
+// site per thread per class).  The packed 16-bit labels and the Philox
+// counter label of the next class are loaded into registers while the
+// current class updates; classes are ordered per wave (wdep) or by a
+// barrier.  The field (FP64, slot order), the draw and the deterministic
+// acceptance are those of ising_sweep_kernel, so the chain is bit-identical
+// to it.  The energy's exact integer bond counts per coupling class come from
+// the half-shell label table (each bond once, its class per slot in icls)
+// when the host has verified that table against the bond list, else from the
+// bond list.
+template <int ZC>
+__global__ __launch_bounds__(kThreads) void ising_sweep_fast(const SweepArgs a) {
+  constexpr int ZH = (ZC + 1) / 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ long long iscr[(kBCMAX + 1) * 16];
+  __shared__ int wprog[16];
+  const int c = blockIdx.x;
+  const int N = a.cp.N, Npad = a.cp.Npad;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int8_t *g = (int8_t *)a.spins + (size_t)c * Npad;
+  int8_t *s = (int8_t *)smem;
+  for (int i = tid; i < Npad; i += blockDim.x) s[i] = g[i];
+  const int slot = a.slot_of_config[c];
+  const double beta = a.beta_slot[slot];
+  const uint32_t step = sweep_of(a) * 16u;
+  const int nc = a.n_colours;
+  const int total = a.metro_passes > 0 ? nc : 0;
+  const int Z = a.cp.Z;
+  const double h0 = a.cp.h_on ? -a.cp.h : 0.0;
+  const bool wsync = a.wdep != nullptr;
+  const __amdgpu_buffer_rsrc_t nrsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.nb16, (short)0, 0x7fffffff, 0x00020000);
+  const __amdgpu_buffer_rsrc_t prsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.perm, (short)0, 0x7fffffff, 0x00020000);
+  uint32_t q[ZH], qn[ZH];
+  int lab = 0, labn = 0;
+  // lanes past a class read row 0 and do no update
+  auto fetch = [&](uint32_t (&d)[ZH], int &l, int b0, int b1) __attribute__((always_inline)) {
+    const int p = b0 + tid;
+    const int pc4 = 4 * (p < b1 ? p : 0);
+#pragma unroll
+    for (int k = 0; k < ZH; k++) d[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrsrc, pc4, 4 * k * N, 0);
+    l = __builtin_amdgcn_raw_buffer_load_b32(prsrc, pc4, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < ZH; k++) q[k] = qn[k] = 0u;
+  if (total > 0) fetch(q, lab, a.col_off[0], a.col_off[1]);
+  if (tid < 16) wprog[tid] = 0;
+  const uint32_t dmv = (wsync && lane < nc) ? a.wdep[lane * 16 + wv] : 0u;
+  __syncthreads();
+  for (int col = 0; col < total; col++) {
+    const int cb = a.col_off[col], ce = a.col_off[col + 1];
+    if (col + 1 < total) fetch(qn, labn, a.col_off[col + 1], a.col_off[col + 2]);
+    if (wsync) {
+      // the ordering of heis_sweep_fast: wait until every wave owning a
+      // neighbour of this wave's sites has completed class col - 1
+      const uint32_t dm = (uint32_t)__builtin_amdgcn_readlane((int)dmv, col);
+      if (dm) {
+        uint32_t n = 0;
+        while (true) {
+          const int v = __hip_atomic_load(&wprog[lane & 15], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const bool behind = ((dm >> (lane & 15)) & 1u) && v < col;
+          if (__builtin_amdgcn_ballot_w64(behind) == 0) break;
+          if (++n > (1u << 20)) {
+            if (lane == 0) atomicOr(a.sync_err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    const int p = cb + tid;
+    if (p < ce) {
+      // every neighbour read issued before the sums, which run in slot order
+      int8_t nv[2 * ZH];
+#pragma unroll
+      for (int k = 0; k < ZH; k++) {
+        nv[2 * k] = s[q[k] & 0xffffu];
+        nv[2 * k + 1] = s[q[k] >> 16];
+      }
+      const int8_t sp = s[p];
+      double h = h0;
+#pragma unroll
+      for (int k = 0; k < ZC; k++)
+        if (k < Z) h += a.cp.jxy[k] * (double)nv[k];
+      const double spin = (double)sp;
+      const double dE = 2.0 * spin * h;
+      const u32x4 r = draw4(a.seed, (uint32_t)lab, step, (uint32_t)c, STREAM_MC_UPDATE);
+      const double u = u01_f64(r.x, r.y);
+      if (accept_exp(u, beta * dE)) s[p] = (int8_t)(-sp);
+    }
+    if (wsync) {
+      if (lane == 0) __hip_atomic_store(&wprog[wv], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < ZH; k++) q[k] = qn[k];
+    lab = labn;
+  }
+  if (wsync) __syncthreads();
+  // exact energy: integer bond counts per coupling class
+  long long cnt[kBCMAX + 1];
+#pragma unroll
+  for (int k = 0; k <= kBCMAX; k++) cnt[k] = 0;
+  if (a.itable) {
+    const __amdgpu_buffer_rsrc_t ersrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.nbE, (short)0, 0x7fffffff, 0x00020000);
+    const int nEw = a.nEw;
+    int acc[16], m = 0;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0;
+    for (int p = tid; p < N; p += blockDim.x) {
+      uint32_t w[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        w[k] = k < nEw ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ersrc, 4 * p, 4 * k * N, 0)
+                       : (uint32_t)N | ((uint32_t)N << 16);
+      const int si = s[p];
+      m += si;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        acc[2 * k] += si * (int)s[w[k] & 0xffffu];
+        acc[2 * k + 1] += si * (int)s[w[k] >> 16];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const int k = a.icls[e];
+#pragma unroll
+      for (int kk = 0; kk < kBCMAX; kk++)
+        if (kk == k) cnt[kk] += acc[e];
+    }
+    cnt[kBCMAX] = m;
+  } else {
+    for (int b = tid; b < a.bd.Nb; b += blockDim.x) {
+      const int k = a.bc[b];
+      const long long pr = (long long)(s[a.bi[b]] * s[a.bj[b]]);
+#pragma unroll
+      for (int kk = 0; kk < kBCMAX; kk++)
+        if (kk == k) cnt[kk] += pr;
+    }
+    for (int p = tid; p < N; p += blockDim.x) cnt[kBCMAX] += s[p];
+  }
+  {
+    const int w = tid >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k <= kBCMAX; k++) cnt[k] = wave_sum_ll(cnt[k]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k <= kBCMAX; k++) iscr[k * 16 + w] = cnt[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k <= kBCMAX; k++) {
+      long long t = 0;
+      for (int i = 0; i < nw; i++) t += iscr[k * 16 + i];
+      cnt[k] = t;
+    }
+  }
+  if (tid == 0) {
+    double E = 0.0;
+#pragma unroll
+    for (int k = 0; k < kBCMAX; k++)
+      if (k < a.bd.n_class) E = fma(a.bd.jxy[k], (double)cnt[k], E);
+    if (a.cp.h_on) E = fma(-a.cp.h, (double)cnt[kBCMAX], E);
+    a.energy[c] = E;
+  }
+  for (int i = tid; i < Npad; i += blockDim.x) g[i] = s[i];
+}
+
 // ------------------------------------------- measurement + replica exchange
 
 struct PTArgs {
@@ -1251,6 +1431,7 @@ struct MCEngine {
   // register-label kernel needs every colour class to fit one pass of them
   int threads = kThreads;
   bool fast_ok = false;
+  bool ifast_ok = false, ifast = false;   // fast Ising kernel possible / in use
   bool compact = false;   // fast kernels read compact labels (see SweepArgs::wcnt)
   DevBuf<uint8_t> wcnt;
   DevBuf<uint32_t> wdep, sync_err;   // wave-level ordering of the fast kernels
@@ -1453,6 +1634,82 @@ static void build_wave_deps(MCEngine &e, const std::vector<uint32_t> &pk, int wo
   a.sync_err = e.sync_err.p;
 }
 
+// The fast kernels' neighbour labels: (ZC + 1) / 2 words per site for the
+// slot class ZC >= Z, two 16-bit labels per word, slots past Z (and holes)
+// holding the sentinel N.
+static std::vector<uint32_t> pack_labels16(const Lattice &lat) {
+  const int N = lat.N, ZH = (lat.Z + 1) / 2, ZHC = (zc_for(lat.Z) + 1) / 2;
+  std::vector<uint32_t> pk((size_t)ZHC * N, (uint32_t)N | ((uint32_t)N << 16));
+  for (int k = 0; k < ZH; k++)
+    for (int p = 0; p < N; p++) {
+      const uint32_t lo = (uint32_t)lat.g_nb[(size_t)(2 * k) * N + p];
+      const uint32_t hi = (2 * k + 1 < lat.Z) ? (uint32_t)lat.g_nb[(size_t)(2 * k + 1) * N + p] : (uint32_t)N;
+      pk[(size_t)k * N + p] = lo | (hi << 16);
+    }
+  return pk;
+}
+
+// Energy table of the fast Ising kernel: the half-shell slots (the first half
+// of each shell's slots), each slot's bond class found by its couplings (bond
+// classes are keyed by (Jxy, Jz)).  Used only when the table's bonds, with
+// those classes, are exactly the bond list's as a multiset: then its integer
+// counts per class equal the bond list's and so does the energy.
+static void ising_energy_table(MCEngine &e) {
+  const Lattice &lat = e.lat;
+  SweepArgs &a = e.args;
+  a.itable = 0;
+  for (int k = 0; k < 16; k++) a.icls[k] = -1;
+  const int N = lat.N;
+  int count[4] = {0, 0, 0, 0};
+  for (int s = 0; s < lat.Z; s++) {
+    if (lat.slot_shell[s] < 1 || lat.slot_shell[s] > 3) return;
+    count[lat.slot_shell[s]]++;
+  }
+  std::vector<int> sl, cls;
+  for (int s = 0; s < lat.Z; s++) {
+    const int sh = lat.slot_shell[s], n = count[sh];
+    if (n % 2) return;
+    if (lat.slot_k[s] >= n / 2) continue;
+    const double jxy = lat.shellJ[sh - 1], jz = lat.shellJ[sh - 1] * lat.shellLam[sh - 1];
+    int k = -1;
+    for (size_t c = 0; c < lat.bc_Jxy.size(); c++)
+      if (lat.bc_Jxy[c] == jxy && lat.bc_Jz[c] == jz) k = (int)c;
+    if (k < 0) return;
+    sl.push_back(s);
+    cls.push_back(k);
+  }
+  if (sl.empty() || sl.size() > 16) return;
+  typedef std::tuple<int, int, int> Bond;
+  std::vector<Bond> tb, tt;
+  tb.reserve(lat.g_bi.size());
+  for (size_t b = 0; b < lat.g_bi.size(); b++) {
+    const int i = lat.g_bi[b], j = lat.g_bj[b];
+    tb.emplace_back(std::min(i, j), std::max(i, j), (int)lat.g_bc[b]);
+  }
+  for (size_t k = 0; k < sl.size(); k++)
+    for (int p = 0; p < N; p++) {
+      const int q = lat.g_nb[(size_t)sl[k] * N + p];
+      if (q < N) tt.emplace_back(std::min(p, q), std::max(p, q), cls[k]);
+    }
+  std::sort(tb.begin(), tb.end());
+  std::sort(tt.begin(), tt.end());
+  if (tb != tt) return;
+  const int nw = ((int)sl.size() + 1) / 2;
+  std::vector<uint32_t> pe((size_t)nw * N, (uint32_t)N | ((uint32_t)N << 16));
+  for (int k = 0; k < nw; k++)
+    for (int p = 0; p < N; p++) {
+      const uint32_t lo = (uint32_t)lat.g_nb[(size_t)sl[2 * k] * N + p];
+      const uint32_t hi = 2 * k + 1 < (int)sl.size() ? (uint32_t)lat.g_nb[(size_t)sl[2 * k + 1] * N + p]
+                                                      : (uint32_t)N;
+      pe[(size_t)k * N + p] = lo | (hi << 16);
+    }
+  for (size_t k = 0; k < cls.size(); k++) a.icls[k] = cls[k];
+  e.nbE.upload(pe.data(), pe.size());
+  a.nbE = e.nbE.p;
+  a.nEw = nw;
+  a.itable = 1;
+}
+
 MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_spec &sp) {
   if (R < 1 || R > kThreads) throw Error(HSMD_ERR_ARG, "replica count must be in [1, 1024]");
   if (sp.model != HSMD_HEISENBERG && sp.model != HSMD_ISING) throw Error(HSMD_ERR_ARG, "unknown model");
@@ -1602,14 +1859,7 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
     if (e->fast_ok) {
       // (ZC + 1) / 2 words per site for the kernel's slot class ZC >= Z;
       // slots past Z hold the sentinel N on both halves
-      const int ZH = (lat.Z + 1) / 2, ZHC = (zc_for(lat.Z) + 1) / 2;
-      std::vector<uint32_t> pk((size_t)ZHC * N, (uint32_t)N | ((uint32_t)N << 16));
-      for (int k = 0; k < ZH; k++)
-        for (int p = 0; p < N; p++) {
-          const uint32_t lo = (uint32_t)lat.g_nb[(size_t)(2 * k) * N + p];
-          const uint32_t hi = (2 * k + 1 < lat.Z) ? (uint32_t)lat.g_nb[(size_t)(2 * k + 1) * N + p] : (uint32_t)N;
-          pk[(size_t)k * N + p] = lo | (hi << 16);
-        }
+      std::vector<uint32_t> pk = pack_labels16(lat);
       // sparse tables (irregular graphs, where most slots hold the
       // sentinel): compact labels, so a wave visits only as many label words
       // as its sites' largest neighbour count needs
@@ -1686,6 +1936,19 @@ MCEngine *mc_create(const Lattice &L, int R, const double *temps, const hsmd_mc_
         a.nbE = e->nbE.p;
         a.nEw = nw;
       }
+    }
+  }
+  {
+    // fast Ising kernel: register labels, wave ordering, table energy
+    e->ifast_ok = e->use_lds && e->ising && !lat.general && (lat.N + 1) < 65535 && e->maxcol <= kThreads;
+    e->ifast = e->ifast_ok;
+    a.itable = 0;
+    if (e->ifast_ok) {
+      std::vector<uint32_t> pk = pack_labels16(lat);
+      e->nb16.upload(pk.data(), pk.size());
+      a.nb16 = e->nb16.p;
+      build_wave_deps(*e, pk, (zc_for(lat.Z) + 1) / 2);
+      ising_energy_table(*e);
     }
   }
   if (e->use_lds) HSMD_ZC_DISPATCH(lat.Z, set_lds_attr, *e);
@@ -1785,7 +2048,8 @@ static void launch_sweep_zc(MCEngine &e, hipStream_t st) {
   a.config_base = 0;
   const size_t sh = e.use_lds ? e.lds_bytes : 0;
   if (e.ising) {
-    if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    if (e.ifast) hipLaunchKernelGGL((ising_sweep_fast<ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
+    else if (e.use_lds) hipLaunchKernelGGL((ising_sweep_kernel<true, ZC>), dim3(e.R), dim3(e.threads), sh, st, a);
     else hipLaunchKernelGGL((ising_sweep_kernel<false, ZC>), dim3(e.R), dim3(e.threads), 0, st, a);
   } else if (e.fast && e.compact && a.n_or + a.n_hb + a.n_or_b + a.n_hb_b > 0) {
     if (f32 && e.aos)
@@ -1870,7 +2134,11 @@ std::string mc_sweep_kernel(const MCEngine &e) {
     return b;
   }
   bool fast = false;
-  if (e.ising) {
+  if (e.ising && e.ifast) {
+    snprintf(b, sizeof b, "ising_sweep_fast<%d> x %d%s", ZC, e.threads,
+             a.itable ? ", table energy" : ", bond-list energy");
+    fast = true;
+  } else if (e.ising) {
     snprintf(b, sizeof b, "ising_sweep_kernel<%s,%d> x %d", e.use_lds ? "true" : "false", ZC, e.threads);
   } else if (e.stat && e.fast && !e.compact && passes && e.lat.Z > 16 && e.lat.Z <= 18) {
     snprintf(b, sizeof b, "heis_sweep_fast<float,18,true,false,true%s> x %d",
@@ -2139,6 +2407,7 @@ void mc_set_threads(MCEngine &e, int threads) {
     throw Error(HSMD_ERR_ARG, "threads per workgroup must be a multiple of 64 in [64, 1024]");
   e.threads = threads;
   e.fast = e.fast_ok && e.maxcol <= threads;
+  e.ifast = e.ifast_ok && e.maxcol <= threads;
   if (e.gexec) { HIP_CHECK(hipGraphExecDestroy(e.gexec)); e.gexec = nullptr; }
   if (e.graph) { HIP_CHECK(hipGraphDestroy(e.graph)); e.graph = nullptr; }
   e.gkey[0] = e.gkey[1] = e.gkey[2] = -9;

@@ -189,3 +189,72 @@ def test_small_lattices_wave_ordered_rk4_equals_barrier_rk4(monkeypatch, prec):
     idx = np.array([0, 1, 1, 0], np.int32)
     samples = [_unit(rng, (lats[i].N, 3)) for i in idx]
     _smd_compare(monkeypatch, lats, idx, samples, 301, n_omega=32, precision=prec)
+
+
+# ------------------------------------------------------------ Ising
+
+def test_config3_ising_fast_kernel_wave_ordered_equals_barrier_and_oracle(monkeypatch):
+    """The Ising variant of configuration 3 (int8 spins, h = 0.3) on the fast
+    Ising kernel: the energy from the half-shell table (verified equal to the
+    bond list as a multiset), classes ordered per wave or by a barrier; both
+    bit-identical, and the energy trace, slot map and spins bit-exact
+    against the oracle's colour replay over four sweeps with exchange."""
+    import cpu_ref
+    _say("config3 ising fast")
+    c = H.Couplings(J1=1.0, J2=0.12, JX=0.05, h=0.3, lambda1=0.9, lambda2=0.9)
+    lat = H.Lattice.generate("triang", 96, c)
+    temps = np.geomspace(0.02, 2.0, 256)
+    rng = np.random.default_rng(413)
+    s = np.zeros((256, lat.N, 3))
+    s[:, :, 0] = rng.choice([-1.0, 1.0], size=(256, lat.N))
+    a = _compare_ising(monkeypatch, lat, temps, s, 4, model=H.ISING, precision=H.BATCHED_F32, seed=29)
+    assert a[0].startswith("ising_sweep_fast<20> x 1024, table energy"), a[0]
+    m = cpu_ref.Model(lat.tables(), (lat.zn1, lat.zn2, lat.znx), c.as_params())
+    ref = s.copy()
+    rtr, (cos, cnt, exc, _) = m.mc_replay(1, temps, lat.colours(), ref, 4, seed=29, pt_parity=2)
+    np.testing.assert_array_equal(a[3], rtr)
+    np.testing.assert_array_equal(a[4], cos)
+    np.testing.assert_array_equal(a[1], ref[cos])
+    np.testing.assert_array_equal(a[5][1], exc)
+
+
+def _compare_ising(monkeypatch, lat, temps, spins, n_sweeps, **kw):
+    a = _chain(monkeypatch, "1", lat, temps, spins, n_sweeps, None, **kw)
+    b = _chain(monkeypatch, "0", lat, temps, spins, n_sweeps, None, **kw)
+    _say(a[0], "|", b[0])
+    assert a[0].endswith("wave-ordered") and not b[0].endswith("wave-ordered")
+    for x, y in zip(a[1:5], b[1:5]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[5][0], b[5][0])
+    np.testing.assert_array_equal(a[5][1], b[5][1])
+    return a
+
+
+@pytest.mark.parametrize("geom,L,kw", [("square", 4, dict(J1=1.0, J2=0.6, JX=0.3)),
+                                       ("square", 24, dict(J1=1.0, J2=0.6)),
+                                       ("kagome", 6, dict(J1=1.0, J2=0.4, h=0.2)),
+                                       ("hexagn", 6, dict(J1=1.0, J2=0.3, JX=0.2))])
+def test_ising_fast_kernel_small_lattices_bit_exact_against_oracle(geom, L, kw):
+    """The fast Ising kernel on small lattices, among them square L=4 with a
+    third shell (whose doubled PBC neighbours leave the half-shell table
+    unequal to the bond list, so the bond-list energy stays): 16 replicas,
+    six sweeps with exchange, bit-exact against the oracle's colour replay."""
+    import cpu_ref
+    c = H.Couplings(**kw)
+    lat = H.Lattice.generate(geom, L, c)
+    temps = np.geomspace(0.3, 4.0, 16)
+    rng = np.random.default_rng(414)
+    s = np.zeros((16, lat.N, 3))
+    s[:, :, 0] = rng.choice([-1.0, 1.0], size=(16, lat.N))
+    mc = H.MonteCarlo(lat, temps, model=H.ISING, precision=H.BATCHED_F32, seed=31)
+    name = mc.sweep_kernel()
+    _say(geom, L, name)
+    assert name.startswith("ising_sweep_fast"), name
+    mc.set_spins(s)
+    tr = mc.run(6, pt_parity=2, trace=True)
+    m = cpu_ref.Model(lat.tables(), (lat.zn1, lat.zn2, lat.znx), c.as_params())
+    ref = s.copy()
+    rtr, (cos, cnt, exc, _) = m.mc_replay(1, temps, lat.colours(), ref, 6, seed=31, pt_parity=2)
+    np.testing.assert_array_equal(tr, rtr)
+    np.testing.assert_array_equal(mc.slot_map(), cos)
+    np.testing.assert_array_equal(mc.get_spins(), ref[cos])

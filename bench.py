@@ -422,7 +422,14 @@ def valu_roofline(profile, kernel_s, hbm_alg_bytes, kernel, unit="launch", extra
     launch's duration divided by its steps; the profile's counts are divided
     by the steps of the profiled launch), so the two need not run equally
     many steps per launch."""
-    vi = profile_value(profile, "valu_insts_per_" + unit)
+    # kernels that order their phases per wave also issue the instructions of
+    # their progress-count waits: the roofline rates the arithmetic alone
+    # (counted with a workgroup barrier per phase) and reports the issued
+    # count beside it
+    issued = profile_value(profile, "valu_insts_per_" + unit)
+    vi = profile_value(profile, "valu_work_insts_per_" + unit)
+    if vi is None:
+        vi = issued
     traffic = profile_value(traffic_profile or profile, "traffic_bytes_per_" + unit)
     lds = profile_value(profile, "lds_wait_frac")
     per = 1.0
@@ -432,6 +439,8 @@ def valu_roofline(profile, kernel_s, hbm_alg_bytes, kernel, unit="launch", extra
            "frac": None if ach is None else ach / VALU_PEAK_GINST,
            "traffic": traffic, "kernel": kernel, "kernel_ms": kernel_s * 1e3,
            "lds_wait_frac": lds,
+           "valu_issued": None if issued is None else
+           {"insts_per_" + unit: issued, "frac": issued / kernel_s / 1e9 / VALU_PEAK_GINST},
            "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": hbm_gbs / HBM_PEAK_GBS,
                    "algorithmic_bytes_per_launch": hbm_alg_bytes,
@@ -556,11 +565,21 @@ def bench_mc(timer, args, rng, tune):
     timer.start()
     mi.run(isw, pt_parity=2)
     ims = timer.stop()
+    # the sweep kernel alone: back-to-back launches, no exchange kernel
+    iks = max(10, args.steps // 2)
+    timer.start()
+    mi.sweeps_only(iks)
+    ikms = timer.stop() / iks
+    # HBM: one read and one write of each int8 replica, the packed labels
+    # ((z + 1) / 2 words per site), the counter labels and the energy's
+    # half-shell labels ((z / 2 + 1) / 2 words per site)
+    ihbm = R * 2 * (N + 1) + 4 * ((z + 1) // 2 + 1 + (z // 2 + 1) // 2) * N
     out["ising"] = {"value": R * N * isw / (ims * 1e-3), "unit": "spin-updates/s",
                     "ms_per_step": ims / isw, "steps": isw,
                     "workload": "configuration 3 graph, 256 replicas, one Metropolis pass + "
                                 "replica exchange per sweep, int8 spins",
-                    "kernel": mi.sweep_kernel()}
+                    "roofline": valu_roofline("mc_ising_sweep_triang96_r256_valu_pmc.txt", ikms * 1e-3,
+                                              ihbm, mi.sweep_kernel())}
     print("[bench] MC ising: %.3f ms/sweep" % (ims / isw), file=sys.stderr, flush=True)
     del mi
     return out, (c, lat, temps)

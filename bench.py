@@ -140,6 +140,34 @@ def smd_workload(rng):
     return lats, idx, samples
 
 
+def mc_error_vs_fp64(lat, temps, rng):
+    """Accuracy of the batched FP32 sweep against the batched FP64 engine
+    on the same replicas: one overrelaxation pass of every replica (no random
+    draws, so both forms update the same chain) from the same unit spins,
+    and the FP64 energies each engine reports for the result.  Heat-bath
+    passes draw in FP32 and FP64 differently and are compared statistically
+    in the tests instead."""
+    R, N = len(temps), lat.N
+    s0 = unit_spins(rng, (R, N, 3))
+    res = {}
+    for prec in (H.BATCHED_F32, H.BATCHED_F64):
+        m = H.MonteCarlo(lat, temps, precision=prec, n_or=1, n_hb=0, n_or_boost=1, n_hb_boost=0, seed=3)
+        m.set_spins(s0)
+        m.run(1, pt_parity=-1)
+        res[prec] = (m.get_spins(), m.energies())
+        del m
+    (s32, e32), (s64, e64) = res[H.BATCHED_F32], res[H.BATCHED_F64]
+    d = np.abs(s32 - s64).max(axis=-1)
+    err = {"workload": "one overrelaxation pass of the %d replicas from the same unit spins" % R,
+           "max_abs_spin_err": float(d.max()), "p9999_abs_spin_err": float(np.quantile(d, 0.9999)),
+           # random spins have energies near zero, so the energy error is
+           # stated per site rather than relative
+           "max_energy_err_per_site": float(np.abs(e32 - e64).max() / N)}
+    print("[bench] MC FP32 vs FP64 (one overrelaxation pass): max |dS| %.2e, energy %.2e per site"
+          % (err["max_abs_spin_err"], err["max_energy_err_per_site"]), file=sys.stderr, flush=True)
+    return err
+
+
 def bench_measure(timer, mc, lat, R, sweep_ms, k):
     """Measurement-stage sweeps on the headline engine: order parameters,
     spin-stiffness bond sums and per-slot moments every sweep (obs_kernel and
@@ -522,6 +550,7 @@ def bench_mc(timer, args, rng, tune):
         "roofline": roof,
     }
     out["measure"] = bench_measure(timer, mc, lat, R, ms / args.steps, max(10, args.steps // 4))
+    out["mc_error_vs_fp64"] = mc_error_vs_fp64(lat, temps, rng)
     del mc
     # the same workload with FP64 state, all replicas in one launch
     mb = H.MonteCarlo(lat, temps, precision=H.BATCHED_F64, n_or=5, n_hb=1, n_or_boost=5,

@@ -1162,7 +1162,8 @@ std::string smd_step_kernel(const SMDEngine &e) {
   }
   int lo = 0, hi = 0;
   if (e.f32 && spt <= 4 && !e.one_class && !e.general) {
-    if (e.minZ <= 8 && Z > 8 && Z <= 12) { lo = e.minZ <= 4 ? 4 : 8; hi = 12; }
+    if (e.minZ <= 8 && Z > 8 && Z <= 10) { lo = e.minZ <= 4 ? 4 : 8; hi = 10; }
+    else if (e.minZ <= 8 && Z > 8 && Z <= 12) { lo = e.minZ <= 4 ? 4 : 8; hi = 12; }
     else if (e.minZ <= 4 && Z > 4 && Z <= 8) { lo = 4; hi = 8; }
   }
   if (hi) snprintf(b, sizeof b, "smd_kernel_two<float,%d,%d,%d> x 1024", spt, lo, hi);

@@ -862,6 +862,13 @@ void smd_launch_zc(const SmdLaunch &c, const StepArgs &a) {
   const int Z = c.maxZ;
   // two slot classes in one LDS-resident batch (FP32 only)
   if constexpr (sizeof(T) == 4 && SPT <= 4) {
+    // a largest class of 9 or 10 slots (the hexagonal lattice with J1 and
+    // J2) is summed over 10 slots, not 12
+    if (c.use_lds && !c.one_class && !c.general && c.minZ <= 8 && Z > 8 && Z <= 10) {
+      if (c.minZ <= 4) smd_launch_two<T, SPT, 4, 10>(c, a);
+      else smd_launch_two<T, SPT, 8, 10>(c, a);
+      return;
+    }
     if (c.use_lds && !c.one_class && !c.general && c.minZ <= 8 && Z > 8 && Z <= 12) {
       if (c.minZ <= 4) smd_launch_two<T, SPT, 4, 12>(c, a);
       else smd_launch_two<T, SPT, 8, 12>(c, a);

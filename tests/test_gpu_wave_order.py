@@ -174,7 +174,7 @@ def test_config4_wave_ordered_rk4_equals_barrier_rk4(monkeypatch, form):
     samples = [_unit(rng, (lats[i].N, 3)) for i in idx]
     a = _smd_compare(monkeypatch, lats, idx, samples, 1025, form=form, n_omega=64,
                      precision=H.BATCHED_F32)
-    assert ("smd_kernel_two<float,4,8,12>" in a[0]) == (form == 2), a[0]
+    assert ("smd_kernel_two<float,4,8,10>" in a[0]) == (form == 2), a[0]
 
 
 @pytest.mark.parametrize("prec", [H.BATCHED_F32, H.BATCHED_F64])

@@ -708,6 +708,35 @@ void heis_sweep_fast(const SweepArgs a) {
     const int p = cb + tid;
     // compact labels: the words any site of this wave's 64-site chunk uses
     const int nw = CP ? __builtin_amdgcn_readfirstlane((int)a.wcnt[col * 16 + (tid >> 6)]) : ZH;
+    const bool act = p < ce;
+    // FP32: the site's own spin (written only by this wave) and the
+    // heat-bath draws do not depend on the neighbours, so they are issued
+    // before the wait for the neighbours' owners and overlap its polls; the
+    // own spin's latency hides behind them.  FP64 does both after the field
+    // sums: held across them, they would push its 128 VGPRs into scratch.
+    constexpr bool EARLY = sizeof(T) == 4;
+    T x = T(0), y = T(0), z = T(0);
+    const uint32_t step = sweep16 + (uint32_t)pass;
+    uint32_t w0 = keep_z, w1 = keep_w;
+    u32x4 r64 = {0u, 0u, 0u, 0u};
+    auto draw_early = [&]() __attribute__((always_inline)) {
+      if (pass < nor) return;
+      if constexpr (sizeof(T) == 4) {
+        if (!((col & 1) && keep)) {
+          const u32x4 r = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
+          w0 = r.x; w1 = r.y;
+          keep_z = r.z; keep_w = r.w;
+        }
+      } else {
+        r64 = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
+      }
+    };
+    if constexpr (EARLY) {
+      if (act) {
+        get(p, x, y, z);
+        draw_early();
+      }
+    }
     if (wsync) {
       const uint32_t dm = (uint32_t)__builtin_amdgcn_readlane((int)dmv, col);
       if (dm) {
@@ -727,33 +756,8 @@ void heis_sweep_fast(const SweepArgs a) {
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    if (p < ce) {
+    if (act) {
       T hx = T(0), hy = T(0), hz = hz0;
-      // FP32: the site's own spin is read with the first group of
-      // neighbours, so its latency hides behind the field sums, and the
-      // heat-bath draws (independent of the field) are computed while the
-      // first group of neighbour reads is in flight.  FP64 does both after
-      // the sums: held across them, they would push its 128 VGPRs into
-      // scratch memory.
-      constexpr bool EARLY = sizeof(T) == 4;
-      T x, y, z;
-      if constexpr (EARLY) get(p, x, y, z);
-      const uint32_t step = sweep16 + (uint32_t)pass;
-      uint32_t w0 = keep_z, w1 = keep_w;
-      u32x4 r64 = {0u, 0u, 0u, 0u};
-      auto draw_early = [&]() __attribute__((always_inline)) {
-        if (pass < nor) return;
-        if constexpr (sizeof(T) == 4) {
-          if (!((col & 1) && keep)) {
-            const u32x4 r = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
-            w0 = r.x; w1 = r.y;
-            keep_z = r.z; keep_w = r.w;
-          }
-        } else {
-          r64 = draw4(a.seed, (uint32_t)lc, step, (uint32_t)c, STREAM_MC_UPDATE);
-        }
-      };
-      if constexpr (CP && EARLY) draw_early();
       if constexpr (!CP) {
         // the neighbour reads in groups (10 neighbours in FP32, 4 in FP64),
         // each group issued before its first product (the scheduling
@@ -770,9 +774,6 @@ void heis_sweep_fast(const SweepArgs a) {
               nget(qc[k0 + k], 0, nx[2 * k], ny[2 * k], nz[2 * k]);
               if (2 * (k0 + k) + 1 < ZC) nget(qc[k0 + k], 1, nx[2 * k + 1], ny[2 * k + 1], nz[2 * k + 1]);
             }
-          }
-          if constexpr (EARLY) {
-            if (k0 == 0) draw_early();
           }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

@@ -1,4 +1,5 @@
-"""GPU tests of the fast sweep kernels' wave-level class ordering.
+"""GPU tests of the wave-level ordering of the fast sweep kernels and the
+LDS-resident RK4 kernel.
 
 With a neighbour-owner mask per (colour class, wave), a wave starts a class
 once every wave owning one of its sites' neighbours has finished the class
@@ -6,8 +7,11 @@ before; under a workgroup barrier per class each site reads the same values.
 So the two orderings must give bit-identical chains: spins, FP64 energies,
 slot maps and exchange counters, through the sweeps, the heat-bath draws and
 the replica exchange (the reference's sweep and exchange loop,
-CF/mc_hm_code.cpp:2778-3130).  HSMD_MC_WAVE_SYNC = 1 / 0 forces either
-ordering at engine creation."""
+CF/mc_hm_code.cpp:2778-3130), for the Heisenberg and the Ising kernels; and
+the RK4 stages ordered per wave must give the trajectories, energies, qSS
+and DSSF of a barrier per stage (CF/subrouts.cpp:11734-11902).
+HSMD_MC_WAVE_SYNC / HSMD_SMD_WAVE_SYNC = 1 / 0 force either ordering at
+engine creation."""
 import math
 import sys
 

@@ -1590,9 +1590,10 @@ static void set_lds_attr(MCEngine &e);
 // label the kernel reads for that wave's sites (the packed table, compact or
 // full; the sentinel N belongs to no wave), without the wave itself.  The
 // fast kernels then order the classes per wave instead of per workgroup.
-// Used when the masks are local (on average at most a quarter of the waves;
-// generated lattices in site order: the waves either side), or as
-// HSMD_MC_WAVE_SYNC = 1 / 0 forces.
+// Used unless HSMD_MC_WAVE_SYNC = 0: generated lattices in site order give
+// masks of the waves either side, and even the broad masks of the
+// Poisson-disk graph (whose classes are sorted by degree: 14 of 16 waves)
+// sweep faster than with a barrier per class.
 static void build_wave_deps(MCEngine &e, const std::vector<uint32_t> &pk, int words) {
   const Lattice &lat = e.lat;
   const int N = lat.N, nc = lat.n_colours;
@@ -1617,15 +1618,7 @@ static void build_wave_deps(MCEngine &e, const std::vector<uint32_t> &pk, int wo
       }
       m &= ~(1u << w);
     }
-  size_t bits = 0, used = 0;
-  for (int col = 0; col < nc; col++) {
-    const int nw = (lat.col_off[col + 1] - lat.col_off[col] + 63) / 64;
-    for (int w = 0; w < nw; w++) {
-      bits += __builtin_popcount(dep[(size_t)col * 16 + w]);
-      used++;
-    }
-  }
-  bool on = used > 0 && bits <= 4 * used;   // mean mask size <= 4 waves
+  bool on = N > 0;
   if (const char *ev = getenv("HSMD_MC_WAVE_SYNC")) on = atoi(ev) != 0;
   if (!on) return;
   e.wdep.upload(dep.data(), dep.size());

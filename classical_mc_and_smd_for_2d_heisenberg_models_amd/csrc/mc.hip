@@ -1106,6 +1106,16 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_fast(const SweepArgs a) 
   for (int col = 0; col < total; col++) {
     const int cb = a.col_off[col], ce = a.col_off[col + 1];
     if (col + 1 < total) fetch(qn, labn, a.col_off[col + 1], a.col_off[col + 2]);
+    const int p = cb + tid;
+    // the own spin (written only by this wave) and the draw do not depend on
+    // the neighbours: issued before the wait, they fill its polls
+    int8_t sp = 0;
+    double u = 0.0;
+    if (p < ce) {
+      sp = s[p];
+      const u32x4 r = draw4(a.seed, (uint32_t)lab, step, (uint32_t)c, STREAM_MC_UPDATE);
+      u = u01_f64(r.x, r.y);
+    }
     if (wsync) {
       // the ordering of heis_sweep_fast: wait until every wave owning a
       // neighbour of this wave's sites has completed class col - 1
@@ -1125,7 +1135,6 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_fast(const SweepArgs a) 
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    const int p = cb + tid;
     if (p < ce) {
       // every neighbour read issued before the sums, which run in slot order
       int8_t nv[2 * ZH];
@@ -1134,15 +1143,12 @@ __global__ __launch_bounds__(kThreads) void ising_sweep_fast(const SweepArgs a) 
         nv[2 * k] = s[q[k] & 0xffffu];
         nv[2 * k + 1] = s[q[k] >> 16];
       }
-      const int8_t sp = s[p];
       double h = h0;
 #pragma unroll
       for (int k = 0; k < ZC; k++)
         if (k < Z) h += a.cp.jxy[k] * (double)nv[k];
       const double spin = (double)sp;
       const double dE = 2.0 * spin * h;
-      const u32x4 r = draw4(a.seed, (uint32_t)lab, step, (uint32_t)c, STREAM_MC_UPDATE);
-      const double u = u01_f64(r.x, r.y);
       if (accept_exp(u, beta * dE)) s[p] = (int8_t)(-sp);
     }
     if (wsync) {
